@@ -1,0 +1,173 @@
+/*
+ * replay_hip.h -- C ABI of libreplay_hip, the MI355X (gfx950) engine for
+ * Replay's Monte Carlo SIR/SEIR step loop.
+ *
+ * The reference exposes no library API: its boundary is the `replay`
+ * executable (tool/gpu_cpu_temporal_sim.cu:570-1185) plus the C functions it
+ * shares through extern "C" headers (tool/temporal_contact_matrix.h:59-94,
+ * tool/ncls/ncls/src/intervaldb.h:76-122).  Each entry point below names the
+ * part of that code it replaces.  Conventions:
+ *   - plain C types and caller-allocated buffers, so ctypes / cgo / JNI can
+ *     bind it; no torch or HIP types cross the boundary;
+ *   - every call returns REPLAY_OK (0) or a negative REPLAY_E_* code and never
+ *     exits the process (the reference calls exit() inside, e.g.
+ *     gpu_cpu_temporal_sim.cu:59-67, temporal_contact_matrix.c:308-312);
+ *   - one replay_ctx owns its device buffers; contexts share no globals, so
+ *     different host threads may drive different contexts;
+ *   - state layout at the boundary is the reference's: N x M int32 countdowns,
+ *     column-major, idx = node + sim * N (gpu_cpu_temporal_sim.cu:754).
+ */
+#ifndef REPLAY_HIP_H
+#define REPLAY_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define REPLAY_HIP_ABI_VERSION 1
+
+enum {
+  REPLAY_OK = 0,
+  REPLAY_E_ARG = -1,        /* bad argument / parameter */
+  REPLAY_E_IO = -2,         /* file cannot be opened or read */
+  REPLAY_E_FORMAT = -3,     /* NODE_LIST header missing or malformed */
+  REPLAY_E_NODE_ID = -4,    /* invalid id in NODE_LIST */
+  REPLAY_E_DUP_ID = -5,     /* duplicate id in NODE_LIST */
+  REPLAY_E_NO_EDGES = -6,   /* no valid contact lines */
+  REPLAY_E_COUNT = -7,      /* more contacts parsed than counted (tcm:308-312) */
+  REPLAY_E_NODE_RANGE = -8, /* more distinct endpoints than NODE_LIST ids */
+  REPLAY_E_NOMEM = -9,      /* host or device allocation failed */
+  REPLAY_E_DEVICE = -10,    /* HIP runtime error */
+  REPLAY_E_OVERFLOW = -11,  /* time arithmetic overflow (cu:861-900) */
+  REPLAY_E_UNSUPPORTED = -12, /* input the engine does not handle */
+  REPLAY_E_STATE = -13      /* call out of sequence */
+};
+
+/* Simulation parameters: the reference's argv globals (cu:24-56, 621-699). */
+typedef struct replay_params {
+  int32_t M;                  /* --M, parallel simulations (default 1000) */
+  float step_size;            /* --step-size seconds (default 3600) */
+  int32_t iterations;         /* --iterations (default 42) */
+  float initial_infected;     /* --initial-infected probability */
+  float infect_prob;          /* --infect-prob per hour of contact */
+  int32_t exposed_duration;   /* --exposed-duration seconds */
+  int32_t infectious_duration;/* --infectious-duration seconds */
+  int32_t resistant_duration; /* --resistant-duration seconds */
+  int64_t start_time;         /* --start-time epoch (default 946713600) */
+  int64_t static_network_duration; /* --static-network-duration (default 3600) */
+  uint64_t seed;              /* Philox4x32-10 key (new: the reference seeds by clock) */
+  int32_t device;             /* HIP device ordinal */
+  int32_t windows_per_epoch;  /* slice packing; <= 0 picks automatically */
+  int64_t max_slice_bytes;    /* device budget for packed slices; <= 0: 32 GiB */
+} replay_params;
+
+typedef struct replay_info {
+  int32_t n_nodes;            /* N (NODE_LIST count) */
+  int32_t M;
+  int64_t n_contacts;         /* valid contacts (intervals) */
+  int64_t nlists;             /* hit cap of every window query (cu:988-989) */
+  int32_t iterations;
+  int32_t n_windows;          /* distinct network windows */
+  int32_t n_epochs;           /* packed slice epochs */
+  int32_t windows_per_epoch;
+  int64_t n_entries;          /* packed directed contact entries */
+  int64_t slice_bytes;        /* device bytes of packed slices */
+  int64_t state_bytes;        /* device bytes of per-replica state */
+  int32_t steps_done;         /* iterations run since the last reset */
+  int32_t truncated_windows;  /* windows whose query hits the cap */
+} replay_info;
+
+typedef struct replay_ctx replay_ctx;
+
+/* ABI version (REPLAY_HIP_ABI_VERSION). */
+int32_t replay_abi_version(void);
+
+/* Fills the reference defaults of cu:25-27, 48-56; seed 0, device 0. */
+void replay_params_init(replay_params *p);
+
+/* Parse a graph file, index its contacts, plan the windows, pack the slices
+   into HBM and draw the initial state.  Replaces
+   parse_edgelist_build_intervals (tcm:148-321), build_interval_db_wrapper
+   (tcm:377-392), the per-iteration find_intervals + build_csr_from_intervals
+   (cu:973-1001) and initialize_countdown_vector (cu:73-90, 788-803). */
+int32_t replay_open(const char *graph_path, const replay_params *p,
+                    replay_ctx **out);
+
+/* Same as replay_open for contacts already in memory: node_ids[n_node_list]
+   is the NODE_LIST (NULL means ids 0..n_node_list-1); contact k joins ids
+   src[k], tgt[k] over [start[k], end[k]) in Unix seconds.  Contacts with a
+   start or end of -1 are skipped, as an unparsable time is (tcm:285-288). */
+int32_t replay_open_contacts(int32_t n_node_list, const int64_t *node_ids,
+                             int64_t n_contacts, const int32_t *src,
+                             const int32_t *tgt, const int64_t *start,
+                             const int64_t *end, const replay_params *p,
+                             replay_ctx **out);
+
+/* Sizes and counters of an open context. */
+int32_t replay_get_info(replay_ctx *ctx, replay_info *out);
+
+/* Redraws the initial state (same seed => same draw) and clears the counts. */
+int32_t replay_reset(replay_ctx *ctx);
+
+/* Enqueue `steps` more iterations of the step loop (cu:859-1150) on the
+   context's stream; returns once they are enqueued.  Each iteration runs
+   mask -> exposure -> probability -> draw -> countdown update -> counts in
+   one fused kernel. */
+int32_t replay_run(replay_ctx *ctx, int32_t steps);
+
+/* replay_run, bracketed by HIP events recorded on the context's stream;
+   waits for the steps and stores their device time in *elapsed_ms.  For
+   benchmarks: the events measure exactly the enqueued step kernels. */
+int32_t replay_run_timed(replay_ctx *ctx, int32_t steps, double *elapsed_ms);
+
+/* Wait for all enqueued work of the context. */
+int32_t replay_sync(replay_ctx *ctx);
+
+/* Per-iteration SEIR counts in CSV column order (exposed, infectious,
+   resistant, susceptible) -- the reduction write_simulation_state performs
+   (cu:505-568).  out holds (steps_done + 1) * M * 4 int32; block 0 is the
+   initial state.  Synchronises. */
+int32_t replay_get_counts(replay_ctx *ctx, int32_t *out, int64_t n_out);
+
+/* Countdown state, N x M int32 column-major (cu:1087-1088).  Synchronises. */
+int32_t replay_get_state(replay_ctx *ctx, int32_t *out, int64_t n_out);
+
+/* Timestamp written with iteration k's block (cu:876, 1093-1095); k = -1
+   gives the initial block's time. */
+int64_t replay_iteration_time(replay_ctx *ctx, int32_t k);
+
+/* Inject a uniform stream for parity tests: u[(slot * M + sim) * N + node],
+   slot 0 = initial draw, slot k+1 = iteration k.  n must be
+   (iterations + 1) * M * N.  NULL restores Philox.  Redraws the initial
+   state. */
+int32_t replay_set_uniforms(replay_ctx *ctx, const float *u, int64_t n);
+
+/* CSR of iteration k's network window as build_csr_from_intervals
+   (cu:238-350) would produce it, evaluated on the device from the packed
+   slices.  rowptr[N+1]; colind/val hold up to cap entries; *nnz receives
+   the count (also when cap is too small, then REPLAY_E_ARG). */
+int32_t replay_get_slice(replay_ctx *ctx, int32_t k, int32_t *rowptr,
+                         int32_t *colind, float *val, int64_t cap,
+                         int64_t *nnz);
+
+/* Exposure seconds E = A_k * I of the current state for iteration k's
+   window, as cusparseSpMM (cu:1062-1064) produces it before
+   compute_infection_probability: out[node + sim * N], N * M floats. */
+int32_t replay_get_exposure(replay_ctx *ctx, int32_t k, float *out,
+                            int64_t n_out);
+
+/* Message of the last failed call on ctx (or of the last failed open on
+   this thread when ctx is NULL). */
+const char *replay_last_error(replay_ctx *ctx);
+
+/* Static description of an error code. */
+const char *replay_error_string(int32_t code);
+
+void replay_close(replay_ctx *ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* REPLAY_HIP_H */

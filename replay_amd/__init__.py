@@ -1,0 +1,298 @@
+"""replay_amd -- Python host for libreplay_hip (MI355X / gfx950).
+
+ctypes bindings of include/replay_hip.h.  The reference drives its engine as a
+subprocess (`replay <graph> [flags]`, see run_sim_gui.py:110-141 and
+benchmark/benchmark_with_trials.py:57-63 in the reference); this module gives
+the same engine in-process: open a graph (file or arrays), run iterations on
+the GPU, read back SEIR counts, states, slices and exposures.
+
+There is no CPU fallback: if libreplay_hip.so is missing or no HIP device is
+present, opening a simulation raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass, field
+
+import numpy as np
+
+PKG_DIR = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(PKG_DIR, "lib", "libreplay_hip.so")
+CLI_PATH = os.path.join(PKG_DIR, "bin", "replay")
+
+ABI_VERSION = 1
+
+# error codes (include/replay_hip.h)
+E_ARG, E_IO, E_FORMAT, E_NODE_ID, E_DUP_ID, E_NO_EDGES, E_COUNT, E_NODE_RANGE = \
+    -1, -2, -3, -4, -5, -6, -7, -8
+E_NOMEM, E_DEVICE, E_OVERFLOW, E_UNSUPPORTED, E_STATE = -9, -10, -11, -12, -13
+
+
+class ReplayError(RuntimeError):
+    def __init__(self, code: int, message: str):
+        super().__init__(f"replay_hip error {code}: {message}")
+        self.code = code
+
+
+class _Params(C.Structure):
+    _fields_ = [("M", C.c_int32), ("step_size", C.c_float), ("iterations", C.c_int32),
+                ("initial_infected", C.c_float), ("infect_prob", C.c_float),
+                ("exposed_duration", C.c_int32), ("infectious_duration", C.c_int32),
+                ("resistant_duration", C.c_int32), ("start_time", C.c_int64),
+                ("static_network_duration", C.c_int64), ("seed", C.c_uint64),
+                ("device", C.c_int32), ("windows_per_epoch", C.c_int32),
+                ("max_slice_bytes", C.c_int64)]
+
+
+class _Info(C.Structure):
+    _fields_ = [("n_nodes", C.c_int32), ("M", C.c_int32), ("n_contacts", C.c_int64),
+                ("nlists", C.c_int64), ("iterations", C.c_int32), ("n_windows", C.c_int32),
+                ("n_epochs", C.c_int32), ("windows_per_epoch", C.c_int32),
+                ("n_entries", C.c_int64), ("slice_bytes", C.c_int64),
+                ("state_bytes", C.c_int64), ("steps_done", C.c_int32),
+                ("truncated_windows", C.c_int32)]
+
+
+EXPORTED_SYMBOLS = (
+    "replay_abi_version", "replay_params_init", "replay_open", "replay_open_contacts",
+    "replay_get_info", "replay_reset", "replay_run", "replay_run_timed", "replay_sync",
+    "replay_get_counts",
+    "replay_get_state", "replay_iteration_time", "replay_set_uniforms", "replay_get_slice",
+    "replay_get_exposure", "replay_last_error", "replay_error_string", "replay_close",
+)
+
+_lib = None
+
+
+def library_path() -> str:
+    return LIB_PATH
+
+
+def load_library() -> C.CDLL:
+    """Loads libreplay_hip.so; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"{LIB_PATH} is missing: build it with `python -m replay_amd.build`")
+    L = C.CDLL(LIB_PATH)
+    P = C.POINTER
+    ctx_p = C.c_void_p
+    L.replay_abi_version.restype = C.c_int32
+    L.replay_params_init.argtypes = [P(_Params)]
+    L.replay_params_init.restype = None
+    L.replay_open.argtypes = [C.c_char_p, P(_Params), P(ctx_p)]
+    L.replay_open_contacts.argtypes = [C.c_int32, P(C.c_int64), C.c_int64, P(C.c_int32),
+                                       P(C.c_int32), P(C.c_int64), P(C.c_int64), P(_Params),
+                                       P(ctx_p)]
+    L.replay_get_info.argtypes = [ctx_p, P(_Info)]
+    L.replay_reset.argtypes = [ctx_p]
+    L.replay_run.argtypes = [ctx_p, C.c_int32]
+    L.replay_run_timed.argtypes = [ctx_p, C.c_int32, P(C.c_double)]
+    L.replay_sync.argtypes = [ctx_p]
+    L.replay_get_counts.argtypes = [ctx_p, P(C.c_int32), C.c_int64]
+    L.replay_get_state.argtypes = [ctx_p, P(C.c_int32), C.c_int64]
+    L.replay_iteration_time.argtypes = [ctx_p, C.c_int32]
+    L.replay_iteration_time.restype = C.c_int64
+    L.replay_set_uniforms.argtypes = [ctx_p, P(C.c_float), C.c_int64]
+    L.replay_get_slice.argtypes = [ctx_p, C.c_int32, P(C.c_int32), P(C.c_int32), P(C.c_float),
+                                   C.c_int64, P(C.c_int64)]
+    L.replay_get_exposure.argtypes = [ctx_p, C.c_int32, P(C.c_float), C.c_int64]
+    L.replay_last_error.argtypes = [ctx_p]
+    L.replay_last_error.restype = C.c_char_p
+    L.replay_error_string.argtypes = [C.c_int32]
+    L.replay_error_string.restype = C.c_char_p
+    L.replay_close.argtypes = [ctx_p]
+    L.replay_close.restype = None
+    for name in ("replay_open", "replay_open_contacts", "replay_get_info", "replay_reset",
+                 "replay_run", "replay_run_timed", "replay_sync", "replay_get_counts", "replay_get_state",
+                 "replay_set_uniforms", "replay_get_slice", "replay_get_exposure"):
+        getattr(L, name).restype = C.c_int32
+    if L.replay_abi_version() != ABI_VERSION:
+        raise RuntimeError("libreplay_hip ABI version mismatch")
+    _lib = L
+    return L
+
+
+@dataclass
+class Params:
+    """Simulation parameters; defaults are the reference's (cu:25-27, 48-56)."""
+
+    M: int = 1000
+    step_size: float = 3600.0
+    iterations: int = 42
+    initial_infected: float = 0.0
+    infect_prob: float = 0.0
+    exposed_duration: int = 0
+    infectious_duration: int = 0
+    resistant_duration: int = 0
+    start_time: int = 946713600
+    static_network_duration: int = 3600
+    seed: int = 0
+    device: int = 0
+    windows_per_epoch: int = 0
+    max_slice_bytes: int = 0
+
+    def _c(self) -> _Params:
+        return _Params(self.M, self.step_size, self.iterations, self.initial_infected,
+                       self.infect_prob, self.exposed_duration, self.infectious_duration,
+                       self.resistant_duration, self.start_time, self.static_network_duration,
+                       self.seed, self.device, self.windows_per_epoch, self.max_slice_bytes)
+
+
+def _ptr(a: np.ndarray, ctype):
+    return a.ctypes.data_as(C.POINTER(ctype))
+
+
+class Simulation:
+    """One network with M replicas on one GPU (a replay_ctx)."""
+
+    def __init__(self, ctx: C.c_void_p, params: Params):
+        self._ctx = ctx
+        self.params = params
+        info = self.info()
+        self.n_nodes = info["n_nodes"]
+        self.M = info["M"]
+
+    # construction -------------------------------------------------------
+    @classmethod
+    def from_file(cls, path: str, params: Params) -> "Simulation":
+        L = load_library()
+        ctx = C.c_void_p()
+        p = params._c()
+        rc = L.replay_open(os.fsencode(path), C.byref(p), C.byref(ctx))
+        if rc != 0:
+            raise ReplayError(rc, L.replay_last_error(None).decode())
+        return cls(ctx, params)
+
+    @classmethod
+    def from_contacts(cls, n_nodes: int, src, tgt, start, end, params: Params,
+                      node_ids=None) -> "Simulation":
+        L = load_library()
+        src = np.ascontiguousarray(src, dtype=np.int32)
+        tgt = np.ascontiguousarray(tgt, dtype=np.int32)
+        start = np.ascontiguousarray(start, dtype=np.int64)
+        end = np.ascontiguousarray(end, dtype=np.int64)
+        ids = None if node_ids is None else np.ascontiguousarray(node_ids, dtype=np.int64)
+        ctx = C.c_void_p()
+        p = params._c()
+        rc = L.replay_open_contacts(n_nodes, None if ids is None else _ptr(ids, C.c_int64),
+                                    len(src), _ptr(src, C.c_int32), _ptr(tgt, C.c_int32),
+                                    _ptr(start, C.c_int64), _ptr(end, C.c_int64),
+                                    C.byref(p), C.byref(ctx))
+        if rc != 0:
+            raise ReplayError(rc, L.replay_last_error(None).decode())
+        return cls(ctx, params)
+
+    # lifecycle ----------------------------------------------------------
+    def close(self) -> None:
+        if self._ctx:
+            load_library().replay_close(self._ctx)
+            self._ctx = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def _check(self, rc: int) -> None:
+        if rc != 0:
+            raise ReplayError(rc, load_library().replay_last_error(self._ctx).decode())
+
+    # queries ------------------------------------------------------------
+    def info(self) -> dict:
+        inf = _Info()
+        self._check(load_library().replay_get_info(self._ctx, C.byref(inf)))
+        return {name: getattr(inf, name) for name, _ in _Info._fields_}
+
+    def reset(self) -> None:
+        self._check(load_library().replay_reset(self._ctx))
+
+    def run(self, steps: int | None = None) -> None:
+        if steps is None:
+            steps = self.params.iterations - self.info()["steps_done"]
+        self._check(load_library().replay_run(self._ctx, steps))
+
+    def run_timed(self, steps: int | None = None) -> float:
+        """Runs `steps` iterations and returns their device time in ms."""
+        if steps is None:
+            steps = self.params.iterations - self.info()["steps_done"]
+        ms = C.c_double(0)
+        self._check(load_library().replay_run_timed(self._ctx, steps, C.byref(ms)))
+        return ms.value
+
+    def sync(self) -> None:
+        self._check(load_library().replay_sync(self._ctx))
+
+    @property
+    def steps_done(self) -> int:
+        return self.info()["steps_done"]
+
+    def counts(self) -> np.ndarray:
+        """(steps_done+1, M, 4) int32: exposed, infectious, resistant, susceptible."""
+        k = self.steps_done
+        out = np.zeros((k + 1, self.M, 4), np.int32)
+        self._check(load_library().replay_get_counts(self._ctx, _ptr(out, C.c_int32), out.size))
+        return out
+
+    def state(self) -> np.ndarray:
+        """N*M int32 countdowns, column-major (idx = node + sim*N)."""
+        out = np.zeros(self.n_nodes * self.M, np.int32)
+        self._check(load_library().replay_get_state(self._ctx, _ptr(out, C.c_int32), out.size))
+        return out
+
+    def iteration_time(self, k: int) -> int:
+        return load_library().replay_iteration_time(self._ctx, k)
+
+    def set_uniforms(self, u: np.ndarray | None) -> None:
+        L = load_library()
+        if u is None:
+            self._check(L.replay_set_uniforms(self._ctx, None, 0))
+            return
+        u = np.ascontiguousarray(u, dtype=np.float32).ravel()
+        self._check(L.replay_set_uniforms(self._ctx, _ptr(u, C.c_float), u.size))
+
+    def slice(self, k: int):
+        """CSR (rowptr, colind, val) of iteration k's window, built on the device."""
+        L = load_library()
+        rowptr = np.zeros(self.n_nodes + 1, np.int32)
+        nnz = C.c_int64(0)
+        L.replay_get_slice(self._ctx, k, _ptr(rowptr, C.c_int32), None, None, 0, C.byref(nnz))
+        n = nnz.value
+        col = np.zeros(max(n, 1), np.int32)
+        val = np.zeros(max(n, 1), np.float32)
+        self._check(L.replay_get_slice(self._ctx, k, _ptr(rowptr, C.c_int32),
+                                       _ptr(col, C.c_int32), _ptr(val, C.c_float), n,
+                                       C.byref(nnz)))
+        return rowptr, col[:n], val[:n]
+
+    def exposure(self, k: int) -> np.ndarray:
+        """Exposure seconds (N*M, column-major) of the current state in window k."""
+        out = np.zeros(self.n_nodes * self.M, np.float32)
+        self._check(load_library().replay_get_exposure(self._ctx, k, _ptr(out, C.c_float),
+                                                       out.size))
+        return out
+
+
+def write_summary_csv(path: str, sim: Simulation) -> None:
+    """Summary CSV in the reference format (cu:707-709, 558-561)."""
+    counts = sim.counts()
+    with open(path, "w") as f:
+        f.write("time,simulation_id,exposed,infectious,resistant,susceptible\n")
+        for k in range(counts.shape[0]):
+            t = sim.iteration_time(k - 1)
+            for s in range(counts.shape[1]):
+                c = counts[k, s]
+                f.write(f"{t},{s},{c[0]},{c[1]},{c[2]},{c[3]}\n")
+
+
+__all__ = ["Params", "Simulation", "ReplayError", "load_library", "library_path",
+           "write_summary_csv", "EXPORTED_SYMBOLS", "CLI_PATH", "LIB_PATH"]

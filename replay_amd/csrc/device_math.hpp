@@ -1,0 +1,68 @@
+// device_math.hpp -- Philox4x32-10 and the project exponential, for gfx950.
+//
+// Philox4x32-10 follows Salmon, Moraes, Dror and Shaw, "Parallel random
+// numbers: as easy as 1, 2, 3" (SC'11): ten rounds of two 32x32->64
+// multiplies (0xD2511F53, 0xCD9E8D57) with Weyl key increments
+// (0x9E3779B9, 0xBB67AE85).
+//
+// rp_expf evaluates e^x in double and rounds once to float: Cody-Waite
+// reduction x = k ln2 + r with |r| <= ln2/2, the degree-13 Taylor polynomial
+// of e^r by fused multiply-adds, scaling by 2^k.  Every step is an IEEE
+// operation with one rounding (rint, fma, ldexp, convert), so host and device
+// builds of the same sequence agree bit for bit.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace rp {
+
+__device__ __forceinline__ uint32_t philox_word0(uint64_t seed, uint32_t c0, uint32_t c1,
+                                                 uint32_t c2, uint32_t c3) {
+  uint32_t k0 = static_cast<uint32_t>(seed), k1 = static_cast<uint32_t>(seed >> 32);
+#pragma unroll
+  for (int r = 0; r < 10; ++r) {
+    const uint32_t lo0 = 0xD2511F53u * c0, hi0 = __umulhi(0xD2511F53u, c0);
+    const uint32_t lo1 = 0xCD9E8D57u * c2, hi1 = __umulhi(0xCD9E8D57u, c2);
+    c0 = hi1 ^ c1 ^ k0;
+    c1 = lo1;
+    c2 = hi0 ^ c3 ^ k1;
+    c3 = lo0;
+    k0 += 0x9E3779B9u;
+    k1 += 0xBB67AE85u;
+  }
+  return c0;
+}
+
+// Uniform in [0, 1): top 24 bits of Philox(counter = {node, sim, slot, 0}).
+__device__ __forceinline__ float philox_uniform(uint64_t seed, uint32_t node, uint32_t sim,
+                                                uint32_t slot) {
+  return static_cast<float>(philox_word0(seed, node, sim, slot, 0u) >> 8) * 0x1p-24f;
+}
+
+__device__ __forceinline__ float rp_expf(float x) {
+  if (x != x) return x;
+  if (x > 88.72283935546875f) return __builtin_huge_valf();
+  if (x < -104.0f) return 0.0f;
+  const double xd = static_cast<double>(x);
+  const double kd = rint(xd * 1.4426950408889634074);
+  double r = fma(-kd, 6.93147180369123816490e-01, xd);
+  r = fma(-kd, 1.90821492927058770002e-10, r);
+  double p = 1.0 / 6227020800.0;
+  p = fma(p, r, 1.0 / 479001600.0);
+  p = fma(p, r, 1.0 / 39916800.0);
+  p = fma(p, r, 1.0 / 3628800.0);
+  p = fma(p, r, 1.0 / 362880.0);
+  p = fma(p, r, 1.0 / 40320.0);
+  p = fma(p, r, 1.0 / 5040.0);
+  p = fma(p, r, 1.0 / 720.0);
+  p = fma(p, r, 1.0 / 120.0);
+  p = fma(p, r, 1.0 / 24.0);
+  p = fma(p, r, 1.0 / 6.0);
+  p = fma(p, r, 0.5);
+  p = fma(p, r, 1.0);
+  p = fma(p, r, 1.0);
+  return static_cast<float>(ldexp(p, static_cast<int>(kd)));
+}
+
+}  // namespace rp

@@ -1,0 +1,272 @@
+// replay_main.cpp -- the `replay` executable on top of libreplay_hip.
+//
+// Keeps the command-line and output contract of the reference executable
+// (tool/gpu_cpu_temporal_sim.cu:570-1185):
+//   - positional graph path (last argument not starting with '-', cu:635-638);
+//   - the same flags and defaults (cu:25-27, 48-56, 639-699); a value flag
+//     without a value or any unknown flag prints "Unrecognized option: %s"
+//     and exits 1;
+//   - -q / --disable-debug-prints silences stdout (cu:621-631);
+//   - output files are removed and re-created while parsing flags
+//     (cu:653-666), headers as cu:707-713, rows as cu:553-560, with the
+//     initial block at --start-time and one block per iteration at
+//     start + llround(iter * step) (cu:829-831, 876, 1093-1095);
+//   - the stdout marker line "Starting temporal Monte Carlo simulation for
+//     %d iterations..." (cu:835-836).
+// Differences: --cpu-only and --cpu-threads are accepted for compatibility
+// but the HIP engine always runs; --seed <u64> and --device <n> are added.
+#include <cstdint>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "replay_hip.h"
+
+namespace {
+
+const char *kStateNames[4] = {"susceptible", "incubating", "infectious", "resistant"};
+
+int classify(int32_t v, int32_t upper, int32_t Ed, int32_t Id, int32_t Rd) {
+  const int32_t elapsed = (v <= 0 || v > upper) ? 0 : (upper - v);
+  if (v == 0) return 0;
+  if (Ed > 0 && elapsed < Ed) return 1;
+  if (Id > 0 && elapsed < Ed + Id) return 2;
+  if (Rd > 0 && elapsed < Ed + Id + Rd) return 3;
+  return 0;
+}
+
+// Buffered writer for the CSV rows.
+class Out {
+ public:
+  explicit Out(FILE *f) : f_(f) { buf_.reserve(1 << 20); }
+  void row_summary(long long t, int sim, const int32_t *c4) {
+    char line[128];
+    int n = std::snprintf(line, sizeof line, "%lld,%d,%d,%d,%d,%d\n", t, sim, c4[0], c4[1], c4[2], c4[3]);
+    put(line, n);
+  }
+  void row_node(long long t, int sim, int node, const char *state) {
+    char line[128];
+    int n = std::snprintf(line, sizeof line, "%lld,%d,%d,%s\n", t, sim, node, state);
+    put(line, n);
+  }
+  void flush() {
+    if (!buf_.empty()) std::fwrite(buf_.data(), 1, buf_.size(), f_);
+    buf_.clear();
+    std::fflush(f_);
+  }
+
+ private:
+  void put(const char *s, int n) {
+    buf_.append(s, static_cast<size_t>(n));
+    if (buf_.size() > (1u << 20)) {
+      std::fwrite(buf_.data(), 1, buf_.size(), f_);
+      buf_.clear();
+    }
+  }
+  FILE *f_;
+  std::string buf_;
+};
+
+void usage(const char *argv0) {
+  std::fprintf(stderr, "Usage: %s <graph file> [options]\n", argv0);
+  std::fprintf(stderr, "Options:\n");
+  std::fprintf(stderr, "  --cpu-only             Accepted for compatibility; the HIP engine runs.\n");
+  std::fprintf(stderr, "  --cpu-threads <int>    Accepted for compatibility.\n");
+  std::fprintf(stderr, "  --M <int>              Number of parallel simulations.\n");
+  std::fprintf(stderr, "  --step-size <float>    Time step size.\n");
+  std::fprintf(stderr, "  --iterations <int>     Number of main simulation steps.\n");
+  std::fprintf(stderr, "  --summary-out <file>   Output file for summary state counts.\n");
+  std::fprintf(stderr, "  --node-state-out <file> Output file for per-node states.\n");
+  std::fprintf(stderr, "  --initial-infected <float>     Initial infection probability [0-1]\n");
+  std::fprintf(stderr, "  --infect-prob <float>          Infection prob per hour of exposure [0-1]\n");
+  std::fprintf(stderr, "  --exposed-duration <int>       Seconds before an infected person becomes infectious\n");
+  std::fprintf(stderr, "  --infectious-duration <int>    Seconds a person remains infectious\n");
+  std::fprintf(stderr, "  --resistant-duration <int>     Seconds a recovered person remains resistant\n");
+  std::fprintf(stderr, "  --start-time <epoch>           Unix start time of the simulation.\n");
+  std::fprintf(stderr, "  --static-network-duration <seconds>  Duration of each static contact network window.\n");
+  std::fprintf(stderr, "  [deprecated] --time-step <seconds>   Alias for --static-network-duration.\n");
+  std::fprintf(stderr, "  --seed <u64>           Philox key of the random draws (default 0).\n");
+  std::fprintf(stderr, "  --device <int>         HIP device ordinal (default 0).\n");
+}
+
+}  // namespace
+
+int main(int argc, char **argv) {
+  std::printf("=== replay (HIP, gfx950) temporal epidemic simulation ===\n");
+  if (argc < 2) {
+    usage(argv[0]);
+    return 1;
+  }
+  bool quiet = false;
+  for (int i = 1; i < argc; ++i) {
+    if (!std::strcmp(argv[i], "--disable-debug-prints") || !std::strcmp(argv[i], "-q")) {
+      quiet = true;
+      if (!std::freopen("/dev/null", "w", stdout)) return 1;
+      break;
+    }
+  }
+  replay_params p;
+  replay_params_init(&p);
+  const char *filename = nullptr;
+  FILE *summary_fp = nullptr, *node_fp = nullptr;
+  for (int i = 1; i < argc; ++i) {
+    const char *a = argv[i];
+    const bool has = i + 1 < argc;
+    if (a[0] != '-') {
+      filename = a;
+      continue;
+    }
+    if (!std::strcmp(a, "--cpu-only")) {
+      std::fprintf(stderr, "[INFO] --cpu-only accepted; the HIP engine runs.\n");
+    } else if (!std::strcmp(a, "--disable-debug-prints") || !std::strcmp(a, "-q")) {
+      quiet = true;
+    } else if (!std::strcmp(a, "--cpu-threads") && has) {
+      ++i;
+    } else if (!std::strcmp(a, "--M") && has) {
+      p.M = std::atoi(argv[++i]);
+    } else if (!std::strcmp(a, "--step-size") && has) {
+      p.step_size = static_cast<float>(std::atof(argv[++i]));
+    } else if (!std::strcmp(a, "--iterations") && has) {
+      p.iterations = std::atoi(argv[++i]);
+    } else if (!std::strcmp(a, "--summary-out") && has) {
+      std::remove(argv[i + 1]);
+      summary_fp = std::fopen(argv[++i], "w");
+      if (!summary_fp) {
+        std::fprintf(stderr, "Error: Could not open summary output file.\n");
+        return 1;
+      }
+    } else if (!std::strcmp(a, "--node-state-out") && has) {
+      std::remove(argv[i + 1]);
+      node_fp = std::fopen(argv[++i], "w");
+      if (!node_fp) {
+        std::fprintf(stderr, "Error: Could not open node state output file.\n");
+        return 1;
+      }
+    } else if (!std::strcmp(a, "--initial-infected") && has) {
+      p.initial_infected = static_cast<float>(std::atof(argv[++i]));
+    } else if (!std::strcmp(a, "--infect-prob") && has) {
+      p.infect_prob = static_cast<float>(std::atof(argv[++i]));
+    } else if (!std::strcmp(a, "--exposed-duration") && has) {
+      p.exposed_duration = std::atoi(argv[++i]);
+    } else if (!std::strcmp(a, "--infectious-duration") && has) {
+      p.infectious_duration = std::atoi(argv[++i]);
+    } else if (!std::strcmp(a, "--resistant-duration") && has) {
+      p.resistant_duration = std::atoi(argv[++i]);
+    } else if (!std::strcmp(a, "--start-time") && has) {
+      p.start_time = static_cast<int64_t>(std::atoll(argv[++i]));
+    } else if ((!std::strcmp(a, "--static-network-duration") || !std::strcmp(a, "--time-step")) &&
+               has) {
+      p.static_network_duration = static_cast<int64_t>(std::atoll(argv[++i]));
+    } else if (!std::strcmp(a, "--seed") && has) {
+      p.seed = static_cast<uint64_t>(std::strtoull(argv[++i], nullptr, 10));
+    } else if (!std::strcmp(a, "--device") && has) {
+      p.device = std::atoi(argv[++i]);
+    } else {
+      std::fprintf(stderr, "Unrecognized option: %s\n", a);
+      return 1;
+    }
+  }
+  if (!filename) {
+    std::fprintf(stderr, "Missing contact network file argument.\n");
+    return 1;
+  }
+  if (summary_fp) std::fprintf(summary_fp, "time,simulation_id,exposed,infectious,resistant,susceptible\n");
+  if (node_fp) std::fprintf(node_fp, "time,simulation_id,node_id,state\n");
+
+  replay_ctx *ctx = nullptr;
+  int rc = replay_open(filename, &p, &ctx);
+  if (rc != REPLAY_OK) {
+    const char *msg = replay_last_error(nullptr);
+    if (rc == REPLAY_E_ARG && std::strstr(msg, "static network"))
+      std::fprintf(stderr, "Error: static network duration must be positive.\n");
+    else if (rc == REPLAY_E_ARG && std::strstr(msg, "step size"))
+      std::fprintf(stderr, "Error: step size must be positive.\n");
+    else
+      std::fprintf(stderr, "No intervals read or parse error.\n");
+    std::fprintf(stderr, "[replay_hip] %s (%s)\n", msg, replay_error_string(rc));
+    return 1;
+  }
+  replay_info info;
+  replay_get_info(ctx, &info);
+  std::printf("[INFO] Node count set from NODE_LIST: %d\n", info.n_nodes);
+  std::fprintf(stderr, "Parsed %lld edges (intervals) from graph.\n", static_cast<long long>(info.n_contacts));
+  std::fprintf(stderr, "NCLS built. n=%lld, nlists=%lld.\n", static_cast<long long>(info.n_contacts),
+               static_cast<long long>(info.nlists));
+
+  const int32_t N = info.n_nodes, M = info.M, T = p.iterations;
+  const int32_t upper = p.exposed_duration + p.infectious_duration + p.resistant_duration;
+  std::vector<int32_t> state;
+  Out sum_out(summary_fp ? summary_fp : stdout), node_out(node_fp ? node_fp : stdout);
+  auto write_block = [&](long long t, const int32_t *counts, const int32_t *st) {
+    for (int32_t sim = 0; sim < M; ++sim) {
+      if (node_fp && st)
+        for (int32_t node = 0; node < N; ++node)
+          node_out.row_node(t, sim, node,
+                            kStateNames[classify(st[static_cast<size_t>(sim) * N + node], upper,
+                                                 p.exposed_duration, p.infectious_duration,
+                                                 p.resistant_duration)]);
+      if (summary_fp) sum_out.row_summary(t, sim, counts + static_cast<size_t>(sim) * 4);
+    }
+    if (summary_fp) sum_out.flush();
+    if (node_fp) node_out.flush();
+  };
+
+  std::vector<int32_t> counts(static_cast<size_t>(T + 1) * M * 4);
+  if (summary_fp || node_fp) {
+    if (node_fp) {
+      state.resize(static_cast<size_t>(N) * M);
+      replay_get_state(ctx, state.data(), static_cast<int64_t>(state.size()));
+    }
+    replay_get_counts(ctx, counts.data(), static_cast<int64_t>(M) * 4);
+    write_block(p.start_time, counts.data(), node_fp ? state.data() : nullptr);
+  }
+  std::printf("\n");
+  std::printf("Starting temporal Monte Carlo simulation for %d iterations...\n", T);
+  std::printf("[INFO] Simulation start time: %lld, time step: %.3f seconds, static network window: %lld seconds\n",
+              static_cast<long long>(p.start_time), static_cast<double>(p.step_size),
+              static_cast<long long>(p.static_network_duration));
+  std::fflush(stdout);
+
+  if (node_fp) {
+    // per-iteration state is needed: run one iteration at a time
+    for (int32_t k = 0; k < T && rc == REPLAY_OK; ++k) {
+      rc = replay_run(ctx, 1);
+      if (rc == REPLAY_OK) rc = replay_get_state(ctx, state.data(), static_cast<int64_t>(state.size()));
+      if (rc == REPLAY_OK) rc = replay_get_counts(ctx, counts.data(), static_cast<int64_t>(k + 2) * M * 4);
+      if (rc == REPLAY_OK)
+        write_block(replay_iteration_time(ctx, k), counts.data() + static_cast<size_t>(k + 1) * M * 4,
+                    state.data());
+    }
+  } else {
+    rc = replay_run(ctx, T);
+    if (rc == REPLAY_OK) rc = replay_get_counts(ctx, counts.data(), static_cast<int64_t>(counts.size()));
+    if (rc == REPLAY_OK && summary_fp)
+      for (int32_t k = 0; k < T; ++k)
+        write_block(replay_iteration_time(ctx, k), counts.data() + static_cast<size_t>(k + 1) * M * 4,
+                    nullptr);
+  }
+  if (rc != REPLAY_OK) {
+    std::fprintf(stderr, "[replay_hip] %s (%s)\n", replay_last_error(ctx), replay_error_string(rc));
+    replay_close(ctx);
+    return 1;
+  }
+  if (!quiet) {
+    const int32_t show = M < 5 ? M : 5;
+    for (int32_t k = 0; k < T; ++k) {
+      std::printf("\n--- Iteration %d (sim time %lld) ---\n", k + 1,
+                  static_cast<long long>(replay_iteration_time(ctx, k)));
+      for (int32_t sim = 0; sim < show; ++sim) {
+        const int32_t *c = counts.data() + (static_cast<size_t>(k + 1) * M + sim) * 4;
+        std::printf("Row %d: Exposed=%d, Infectious=%d, Resistant=%d, Susceptible=%d\n", sim, c[0],
+                    c[1], c[2], c[3]);
+      }
+    }
+  }
+  std::printf("Simulation done.\n");
+  replay_close(ctx);
+  if (summary_fp) std::fclose(summary_fp);
+  if (node_fp) std::fclose(node_fp);
+  return 0;
+}

@@ -1,0 +1,259 @@
+"""HIP engine vs the CPU oracle, through the C ABI (needs an MI355X).
+
+Bar: bit-exact for slices (CSR of every window), countdown states and counts;
+exposure bit-exact against the sequential fp32 order and within 1e-6 relative
+of FP64.  Inputs are seeded; the same Philox key drives both sides."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from conftest import ROOT, graph_path
+
+import replay_amd
+from replay_amd import Params, Simulation, workloads
+
+pytestmark = pytest.mark.gpu
+
+FIXTURES = ["readme_example.txt", "edge_cases.txt", "contactgraphgen_graph.txt",
+            "random_flat.txt", "random_nested.txt", "random_dense.txt"]
+
+
+def _to_oracle_params(oracle, p: Params, uniforms=None, threads=1):
+    return oracle.make_params(M=p.M, step_size=p.step_size, iterations=p.iterations,
+                              initial_infected=p.initial_infected, infect_prob=p.infect_prob,
+                              exposed_duration=p.exposed_duration,
+                              infectious_duration=p.infectious_duration,
+                              resistant_duration=p.resistant_duration, start_time=p.start_time,
+                              static_network_duration=p.static_network_duration, seed=p.seed,
+                              threads=threads, uniforms=uniforms)
+
+
+def _graph_start(path):
+    import calendar
+    import time
+    ts = []
+    for line in open(path).read().splitlines()[1:]:
+        parts = [x.strip() for x in line.split(",")]
+        if len(parts) >= 3:
+            try:
+                ts.append(calendar.timegm(time.strptime(parts[2][:19], "%Y-%m-%dT%H:%M:%S")))
+            except ValueError:
+                pass
+    t = min(ts)
+    return t - t % 600
+
+
+def _random_case(seed, n=40, e=600, span=2 * 86400, long_frac=0.2):
+    rng = np.random.default_rng(seed)
+    a = rng.integers(0, n, e)
+    b = (a + 1 + rng.integers(0, n - 1, e)) % n
+    s = 946713600 + rng.integers(0, span, e)
+    d = np.where(rng.uniform(size=e) < long_frac, rng.integers(3600, span, e),
+                 rng.integers(0, 5400, e))
+    dup = rng.uniform(size=e) < 0.05
+    s[1:][dup[1:]] = s[:-1][dup[1:]]
+    d[1:][dup[1:]] = d[:-1][dup[1:]]
+    return n, a.astype(np.int32), b.astype(np.int32), s.astype(np.int64), (s + d).astype(np.int64)
+
+
+STEP_WINDOWS = [(600.0, 600), (600.0, 1800), (3600.0, 3600), (1000.0, 3600), (450.0, 3600)]
+
+
+@pytest.mark.parametrize("name", FIXTURES)
+@pytest.mark.parametrize("step,D", STEP_WINDOWS[:3])
+def test_slices_fixture_graphs(oracle, name, step, D):
+    path = graph_path(name)
+    p = Params(M=2, step_size=step, static_network_duration=D, iterations=48,
+               start_time=_graph_start(path))
+    g = oracle.OracleGraph.from_file(path)
+    with Simulation.from_file(path, p) as sim:
+        assert sim.info()["nlists"] == g.nlists
+        op = _to_oracle_params(oracle, p)
+        for k in range(p.iterations):
+            rp, col, val = sim.slice(k)
+            orp, ocol, oval, _ = g.step_csr(op, k)
+            np.testing.assert_array_equal(rp, orp)
+            np.testing.assert_array_equal(col, ocol)
+            np.testing.assert_array_equal(val.view(np.int32), oval.view(np.int32))
+
+
+@pytest.mark.parametrize("seed", range(4))
+@pytest.mark.parametrize("step,D", STEP_WINDOWS)
+@pytest.mark.parametrize("wpe", [1, 4, 0])
+def test_slices_random(oracle, seed, step, D, wpe):
+    n, a, b, s, e = _random_case(seed)
+    p = Params(M=3, step_size=step, static_network_duration=D, iterations=120,
+               windows_per_epoch=wpe)
+    g = oracle.OracleGraph.from_contacts(n, a, b, s, e)
+    op = _to_oracle_params(oracle, p)
+    with Simulation.from_contacts(n, a, b, s, e, p) as sim:
+        for k in range(0, p.iterations, 3):
+            rp, col, val = sim.slice(k)
+            orp, ocol, oval, _ = g.step_csr(op, k)
+            np.testing.assert_array_equal(rp, orp)
+            np.testing.assert_array_equal(col, ocol)
+            np.testing.assert_array_equal(val.view(np.int32), oval.view(np.int32))
+
+
+@pytest.mark.parametrize("seed", range(3))
+@pytest.mark.parametrize("step,D", [(600.0, 600), (1000.0, 3600)])
+def test_exposure_parity(oracle, seed, step, D):
+    import ctypes as C
+    n, a, b, s, e = _random_case(seed, n=64, e=1500)
+    p = Params(M=130, step_size=step, static_network_duration=D, iterations=60,
+               initial_infected=0.3, infect_prob=0.2, exposed_duration=1200,
+               infectious_duration=7200, resistant_duration=3600, seed=seed)
+    g = oracle.OracleGraph.from_contacts(n, a, b, s, e)
+    op = _to_oracle_params(oracle, p)
+    P = lambda arr, t: arr.ctypes.data_as(C.POINTER(t))
+    with Simulation.from_contacts(n, a, b, s, e, p) as sim:
+        for k in range(0, p.iterations, 7):
+            state = sim.state()
+            got = sim.exposure(k)
+            lower, medium = p.resistant_duration, p.infectious_duration + p.resistant_duration
+            inf = ((state > lower) & (state <= medium)).astype(np.float32)
+            rp, col, val, _ = g.step_csr(op, k)
+            ref = np.zeros(n * p.M, np.float32)
+            ref64 = np.zeros(n * p.M, np.float64)
+            oracle.lib().or_spmm(P(rp, C.c_int32), P(col, C.c_int32), P(val, C.c_float), len(col),
+                                 n, P(inf, C.c_float), P(ref, C.c_float), p.M, 1)
+            oracle.lib().or_spmm_f64(P(rp, C.c_int32), P(col, C.c_int32), P(val, C.c_float), n,
+                                     P(inf, C.c_float), P(ref64, C.c_double), p.M)
+            np.testing.assert_array_equal(got.view(np.int32), ref.view(np.int32))
+            np.testing.assert_allclose(got, ref64, rtol=1e-6, atol=0)
+            sim.run(7 if k + 7 <= p.iterations else p.iterations - k)
+
+
+CASES = [
+    # (M, step, D, E, I, R, p0, beta)
+    (1, 600.0, 600, 0, 7200, 0, 0.2, 0.5),            # SIS-like (R = 0)
+    (63, 600.0, 600, 1800, 7200, 10 ** 6, 0.05, 0.3),  # SEIR, permanent R
+    (64, 1000.0, 3600, 1200, 3600, 7200, 0.1, 0.4),    # SEIRS, fractional weights
+    (65, 3600.0, 3600, 0, 86400, 86400, 0.1, 0.05),    # SIRS
+    (130, 450.0, 3600, 900, 5400, 0, 0.1, 1.0),        # p = 1 (NaN for zero exposure)
+    (200, 1234.5, 1800, 3000, 9000, 20000, 0.05, 0.2), # non-integer step
+]
+
+
+@pytest.mark.parametrize("case", CASES)
+@pytest.mark.parametrize("seed", [0, 7])
+def test_trajectory_parity(oracle, case, seed):
+    M, step, D, Ed, Id, Rd, p0, beta = case
+    n, a, b, s, e = _random_case(seed + 10, n=80, e=2500)
+    p = Params(M=M, step_size=step, static_network_duration=D, iterations=150,
+               initial_infected=p0, infect_prob=beta, exposed_duration=Ed,
+               infectious_duration=Id, resistant_duration=Rd, seed=1000 + seed)
+    g = oracle.OracleGraph.from_contacts(n, a, b, s, e)
+    counts, state, _ = g.simulate(_to_oracle_params(oracle, p), want_state=True)
+    with Simulation.from_contacts(n, a, b, s, e, p) as sim:
+        sim.run(60)
+        sim.run(p.iterations - 60)
+        np.testing.assert_array_equal(sim.counts(), counts)
+        np.testing.assert_array_equal(sim.state(), state)
+        assert counts[:, :, 1].max() > 0
+
+
+def test_injected_uniform_stream(oracle):
+    n, a, b, s, e = _random_case(3, n=50, e=1200)
+    p = Params(M=70, step_size=600.0, static_network_duration=1800, iterations=80,
+               initial_infected=0.1, infect_prob=0.3, exposed_duration=1200,
+               infectious_duration=7200, resistant_duration=3600)
+    rng = np.random.default_rng(5)
+    # a stream in [0, 1] with 15-bit resolution, like rand()/RAND_MAX on Windows
+    u = (rng.integers(0, 32768, size=(p.iterations + 1) * p.M * n) / 32767.0).astype(np.float32)
+    g = oracle.OracleGraph.from_contacts(n, a, b, s, e)
+    counts, state, _ = g.simulate(_to_oracle_params(oracle, p, uniforms=u), want_state=True)
+    with Simulation.from_contacts(n, a, b, s, e, p) as sim:
+        sim.set_uniforms(u)
+        sim.run()
+        np.testing.assert_array_equal(sim.counts(), counts)
+        np.testing.assert_array_equal(sim.state(), state)
+        sim.set_uniforms(None)
+        sim.run()
+        c2 = sim.counts()
+    counts2, _, _ = g.simulate(_to_oracle_params(oracle, p), want_state=False)
+    np.testing.assert_array_equal(c2, counts2)
+
+
+def test_truncated_windows_trajectory(oracle):
+    """Windows whose query exceeds the hit cap (every contact overlapping)."""
+    n = 6
+    a = np.array([0, 1, 2, 3, 4, 0], np.int32)
+    b = np.array([1, 2, 3, 4, 5, 5], np.int32)
+    s = np.full(6, 946713600 - 100, np.int64)
+    e = s + np.array([10 ** 6, 10 ** 6, 10 ** 6 - 1, 10 ** 6, 5000, 10 ** 6])
+    p = Params(M=64, step_size=3600.0, static_network_duration=3600, iterations=40,
+               initial_infected=0.2, infect_prob=0.9, infectious_duration=36000,
+               resistant_duration=36000)
+    g = oracle.OracleGraph.from_contacts(n, a, b, s, e)
+    counts, state, _ = g.simulate(_to_oracle_params(oracle, p), want_state=True)
+    with Simulation.from_contacts(n, a, b, s, e, p) as sim:
+        assert sim.info()["truncated_windows"] > 0
+        sim.run()
+        np.testing.assert_array_equal(sim.counts(), counts)
+        np.testing.assert_array_equal(sim.state(), state)
+
+
+def test_config0_full_size(oracle):
+    w = workloads.config0_sir()
+    g = oracle.OracleGraph.from_contacts(w.n_nodes, w.src, w.tgt, w.start, w.end)
+    counts, state, _ = g.simulate(_to_oracle_params(oracle, w.params, threads=8), want_state=True)
+    with Simulation.from_contacts(w.n_nodes, w.src, w.tgt, w.start, w.end, w.params) as sim:
+        sim.run()
+        np.testing.assert_array_equal(sim.counts(), counts)
+        np.testing.assert_array_equal(sim.state(), state)
+    assert counts[-1, :, 2].mean() > 10  # the epidemic took off
+
+
+def test_config1_reduced(oracle):
+    w = workloads.config1_village(n_nodes=2000, n_contacts=200_000, M=96, iterations=700, days=6)
+    g = oracle.OracleGraph.from_contacts(w.n_nodes, w.src, w.tgt, w.start, w.end)
+    counts, state, _ = g.simulate(_to_oracle_params(oracle, w.params, threads=8), want_state=True)
+    with Simulation.from_contacts(w.n_nodes, w.src, w.tgt, w.start, w.end, w.params) as sim:
+        sim.run()
+        np.testing.assert_array_equal(sim.counts(), counts)
+        np.testing.assert_array_equal(sim.state(), state)
+
+
+def test_config1_full_size_invariants():
+    """Full village configuration: size-independent properties."""
+    w = workloads.config1_village()
+    with Simulation.from_contacts(w.n_nodes, w.src, w.tgt, w.start, w.end, w.params) as sim:
+        sim.run()
+        c = sim.counts()
+        st = sim.state()
+    assert c.shape == (w.params.iterations + 1, w.params.M, 4)
+    assert (c.sum(axis=2) == w.n_nodes).all()
+    # permanent R within the horizon: recovered counts never decrease
+    assert (np.diff(c[:, :, 2], axis=0) >= 0).all()
+    # ever-infected (E + I + R) never decreases
+    assert (np.diff(c[:, :, :3].sum(axis=2), axis=0) >= 0).all()
+    assert ((st == 0) | (st > 0)).all()
+    assert c[-1, :, 2].mean() > 1
+
+
+def _run(cmd):
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr[-2000:]
+    return r
+
+
+def test_cli_csv_byte_identical(oracle, tmp_path):
+    path = graph_path("random_nested.txt")
+    args = [path, "--M", "7", "--step-size", "1200", "--iterations", "40", "--time-step", "3600",
+            "--initial-infected", "0.2", "--infect-prob", "0.4", "--exposed-duration", "3600",
+            "--infectious-duration", "7200", "--resistant-duration", "3600",
+            "--start-time", str(_graph_start(path)), "--seed", "11"]
+    ours = [str(tmp_path / "s1.csv"), str(tmp_path / "n1.csv")]
+    ref = [str(tmp_path / "s2.csv"), str(tmp_path / "n2.csv")]
+    r = _run([replay_amd.CLI_PATH, *args, "--summary-out", ours[0], "--node-state-out", ours[1]])
+    assert "Starting temporal Monte Carlo simulation for 40 iterations..." in r.stdout
+    _run([os.path.join(ROOT, "oracle", "replay_cpu"), *args, "--summary-out", ref[0],
+          "--node-state-out", ref[1]])
+    for x, y in zip(ours, ref):
+        assert open(x).read() == open(y).read()
+    # summary only (the all-iterations-at-once path)
+    _run([replay_amd.CLI_PATH, *args, "-q", "--summary-out", ours[0]])
+    assert open(ours[0]).read() == open(ref[0]).read()
