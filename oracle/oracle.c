@@ -697,10 +697,14 @@ int or_build_csr(const or_interval *hits, int n_hits, int n_nodes, int64_t ws,
 int or_step_csr(const or_graph *g, const or_params *p, int iter, int32_t *rowptr,
                 int32_t *colind, float *val, int64_t *nnz, int64_t *ws_out,
                 int64_t *we_out, float *scale_out) {
+  /* the divisor is fixed at the first iteration of each window (cu:904-957),
+     so the cache is replayed from iteration 0 */
   or_window_cache cache = {0, 0, 1};
   int64_t t, ws, we;
   float scale;
-  int rc = or_window(p, &cache, iter, &t, &ws, &we, &scale);
+  int rc = OR_OK;
+  for (int it = 0; it <= iter && rc == OR_OK; it++)
+    rc = or_window(p, &cache, it, &t, &ws, &we, &scale);
   if (rc) return rc;
   int cap = g->nlists > 0 ? g->nlists : g->n_top;
   or_interval *hits = (or_interval *)malloc(sizeof(or_interval) * (size_t)cap);

@@ -48,12 +48,15 @@ def build(quiet: bool = True) -> str:
     os.makedirs(BUILD, exist_ok=True)
     os.makedirs(LIB_DIR, exist_ok=True)
     os.makedirs(BIN_DIR, exist_ok=True)
-    ingest_o = os.path.join(BUILD, "ingest.o")
+    host_objs = []
+    for src in ("ingest.cpp", "host_slices.cpp"):
+        obj = os.path.join(BUILD, src.replace(".cpp", ".o"))
+        _run(["g++", *CXX_FLAGS, "-c", os.path.join(CSRC, src), "-o", obj], quiet)
+        host_objs.append(obj)
     hip_o = os.path.join(BUILD, "replay_hip.o")
-    _run(["g++", *CXX_FLAGS, "-c", os.path.join(CSRC, "ingest.cpp"), "-o", ingest_o], quiet)
     _run([HIPCC, *HIP_FLAGS, "-c", os.path.join(CSRC, "replay_hip.hip"), "-o", hip_o], quiet)
     tmp = LIB + ".tmp"
-    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-pthread", ingest_o, hip_o,
+    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-pthread", *host_objs, hip_o,
           "-o", tmp], quiet)
     os.replace(tmp, LIB)
     _run(["g++", "-O2", "-std=c++17", f"-I{INCLUDE}", os.path.join(CSRC, "replay_main.cpp"),

@@ -1,0 +1,88 @@
+// host_slices.cpp -- host evaluation of the packed slices, for CPU tests.
+//
+// Not part of the public ABI (include/replay_hip.h): these exports let the
+// test suite check ingestion, window planning and slice packing on a machine
+// without a GPU.  rp_host_slice evaluates one iteration's CSR from the packed
+// entries with the same rules the device slice kernel applies (overlap with
+// the window, rank limit of the hit cap, rank-order merge of one column).
+#include <algorithm>
+#include <cstdint>
+#include <cstring>
+#include <string>
+
+#include "ingest.hpp"
+
+struct rp_host_plan {
+  rp::Network net;
+  rp::Plan plan;
+  rp::Slices slices;
+  std::string err;
+};
+
+extern "C" {
+
+int32_t rp_host_plan_open(int32_t n_node_list, int64_t n, const int32_t *src, const int32_t *tgt,
+                          const int64_t *start, const int64_t *end, const replay_params *p,
+                          rp_host_plan **out) {
+  auto *h = new rp_host_plan();
+  int rc = rp::from_contacts(n_node_list, nullptr, n, src, tgt, start, end, h->net, h->err);
+  if (!rc) rc = rp::finalize(h->net, h->err);
+  if (!rc) rc = rp::plan_windows(h->net, *p, h->plan, h->err);
+  if (!rc)
+    rc = rp::build_slices(h->net, h->plan, p->windows_per_epoch,
+                          p->max_slice_bytes > 0 ? p->max_slice_bytes : (int64_t(1) << 32),
+                          h->slices, h->err);
+  if (rc) {
+    delete h;
+    return rc;
+  }
+  *out = h;
+  return 0;
+}
+
+int64_t rp_host_plan_nlists(rp_host_plan *h) { return h->net.nlists; }
+int32_t rp_host_plan_epochs(rp_host_plan *h) { return static_cast<int32_t>(h->slices.epoch_t0.size()); }
+int32_t rp_host_plan_int_mode(rp_host_plan *h, int32_t k) {
+  return h->plan.windows[h->plan.iter_window[k]].int_mode;
+}
+
+int32_t rp_host_slice(rp_host_plan *h, int32_t k, int32_t *rowptr, int32_t *colind, float *val,
+                      int64_t cap, int64_t *nnz) {
+  const rp::Window &w = h->plan.windows[h->plan.iter_window[k]];
+  const int64_t N = h->net.n_nodes;
+  const int32_t *rp = &h->slices.rowptr[static_cast<size_t>(w.epoch) * (N + 1)];
+  const rp::Entry *en = h->slices.entries.data() + h->slices.epoch_entry0[w.epoch];
+  const int64_t t0 = h->slices.epoch_t0[w.epoch];
+  const int32_t ws = static_cast<int32_t>(w.ws - t0), we = static_cast<int32_t>(w.we - t0);
+  int64_t out = 0;
+  rowptr[0] = 0;
+  for (int64_t i = 0; i < N; ++i) {
+    int32_t e = rp[i];
+    while (e < rp[i + 1]) {
+      const int32_t col = en[e].col;
+      float wt = 0.0f;
+      bool any = false;
+      for (; e < rp[i + 1] && en[e].col == col; ++e) {
+        const int32_t ov = std::min(en[e].te, we) - std::max(en[e].ts, ws);
+        if (ov > 0 && en[e].rank <= w.rank_limit) {
+          const float wc = static_cast<float>(ov) * w.scale;
+          wt = any ? wt + wc : wc;
+          any = true;
+        }
+      }
+      if (any) {
+        if (out >= cap) return -1;
+        colind[out] = col;
+        val[out] = wt;
+        ++out;
+      }
+    }
+    rowptr[i + 1] = static_cast<int32_t>(out);
+  }
+  *nnz = out;
+  return 0;
+}
+
+void rp_host_plan_close(rp_host_plan *h) { delete h; }
+
+}  // extern "C"

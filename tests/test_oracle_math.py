@@ -142,8 +142,12 @@ def test_csr_against_dense_bruteforce(oracle, seed, step, D):
     iv = g.intervals()
     cap = max(g.nlists, 1)
     order = np.lexsort((-iv[:, 1], iv[:, 0]))
+    win = [int(k * step) // D for k in range(T)]  # integer steps: offsets are exact
     for it in range(T):
         rowptr, col, val, (ws, we, sc) = g.step_csr(p, it)
+        # closed form: the divisor is the number of iterations sharing the window
+        assert sc == np.float32(1.0) / np.float32(win.count(win[it]))
+        assert ws == 946713600 + win[it] * D and we == ws + D
         dense = np.zeros((g.n_nodes, g.n_nodes), np.float64)
         got = np.zeros_like(dense)
         for r in range(g.n_nodes):
