@@ -28,13 +28,13 @@ HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, published
 
 
 def step_kernel_bytes(info: dict) -> float:
-    """Algorithmic HBM bytes of one fused step-kernel launch:
-    countdown state read + write (8 B per replica-node), infectious mask read
-    + write (1/8 B each), the epoch's packed entries (16 B each), its row
-    pointers and the counts block (3 int32 per replica)."""
-    N, M = info["n_nodes"], info["M"]
-    entries_per_epoch = info["n_entries"] / max(1, info["n_epochs"])
-    return 8.0 * N * M + N * M / 4.0 + 16.0 * entries_per_epoch + 4.0 * (N + 1) + 12.0 * M
+    """Algorithmic HBM bytes of one step-kernel launch, as the library
+    defines them (replay_info.step_bytes): per 64-replica word the
+    susceptible word read + write, the own infectious word read, the next
+    infectious word write, the iteration's infection word write and one ring
+    read per timer offset; the slice epoch's entries (16 B) and row pointers;
+    the per-replica prefix update (12 B)."""
+    return float(info["step_bytes"])
 
 
 def cpu_baseline(w, threads: int) -> dict:

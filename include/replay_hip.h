@@ -77,6 +77,9 @@ typedef struct replay_info {
   int64_t state_bytes;        /* device bytes of per-replica state */
   int32_t steps_done;         /* iterations run since the last reset */
   int32_t truncated_windows;  /* windows whose query hits the cap */
+  int64_t step_bytes;         /* algorithmic HBM bytes of one step-kernel launch */
+  int32_t ring_depth;         /* iterations of infection bit words kept on device */
+  int32_t reserved;
 } replay_info;
 
 typedef struct replay_ctx replay_ctx;

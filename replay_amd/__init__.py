@@ -51,7 +51,8 @@ class _Info(C.Structure):
                 ("n_epochs", C.c_int32), ("windows_per_epoch", C.c_int32),
                 ("n_entries", C.c_int64), ("slice_bytes", C.c_int64),
                 ("state_bytes", C.c_int64), ("steps_done", C.c_int32),
-                ("truncated_windows", C.c_int32)]
+                ("truncated_windows", C.c_int32), ("step_bytes", C.c_int64),
+                ("ring_depth", C.c_int32), ("reserved", C.c_int32)]
 
 
 EXPORTED_SYMBOLS = (
