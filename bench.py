@@ -1,12 +1,19 @@
 """Benchmark of the HIP Monte Carlo SEIR step loop on one MI355X.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload config1]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload config3|config1|...]
 
 A "step" is one full simulation of the workload: reset (initial draw) plus
-all T iterations of the step loop (fused exposure / draw / countdown / counts
-kernel per iteration) over M replicas.  Inputs (packed contact slices, state)
-are resident in HBM before timing; ingestion is excluded, per-step counts are
-included.  value = N * M * T * K / wall seconds, in replica*node*steps/s.
+all T iterations of the step loop (fused exposure / draw / timer / counts
+kernel per iteration) over every replica.  Inputs (packed contact slices,
+state) are resident in HBM before timing; ingestion is excluded, per-step
+counts are included.  value = sum_k N_k * M * T * K / wall seconds, in
+replica*node*steps/s.
+
+config3 (default): 64 uneven networks.  The batched form (one context, one
+step launch for all networks, the T launches captured in a hipGraph, kernel
+variant picked by the autotuner) is timed against the plain per-network
+launch loop (one context per network, eager per-iteration launches) on the
+same GPU; both must give identical counts.
 
 Prints one JSON line (the driver's contract) with a roofline object for the
 step kernel (algorithmic bytes per launch / HIP-event time per launch) and a
@@ -27,25 +34,19 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, published
 
 
-def step_kernel_bytes(info: dict) -> float:
-    """Algorithmic HBM bytes of one step-kernel launch, as the library
-    defines them (replay_info.step_bytes): per 64-replica word the
-    susceptible word read + write, the own infectious word read, the next
-    infectious word write, the iteration's infection word write and one ring
-    read per timer offset; the slice epoch's entries (16 B) and row pointers;
-    the per-replica prefix update (12 B)."""
-    return float(info["step_bytes"])
+def log(msg: str) -> None:
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def cpu_baseline(w, threads: int) -> dict:
+def cpu_baseline(name, n_nodes, src, tgt, start, end, p, threads: int, M_s: int, T_s: int,
+                 note: str = "") -> dict:
     """Oracle on the host: same network, M and T truncated to a bounded sample."""
     from oracle import oracle_py
     if not os.path.exists(oracle_py.LIB_PATH):
         oracle_py.build()
-    p = w.params
-    M_s = min(p.M, 64)
-    T_s = min(p.iterations, 240)
-    g = oracle_py.OracleGraph.from_contacts(w.n_nodes, w.src, w.tgt, w.start, w.end)
+    M_s = min(p.M, M_s)
+    T_s = min(p.iterations, T_s)
+    g = oracle_py.OracleGraph.from_contacts(n_nodes, src, tgt, start, end)
     op = oracle_py.make_params(M=M_s, step_size=p.step_size, iterations=T_s,
                                initial_infected=p.initial_infected, infect_prob=p.infect_prob,
                                exposed_duration=p.exposed_duration,
@@ -54,11 +55,152 @@ def cpu_baseline(w, threads: int) -> dict:
                                static_network_duration=p.static_network_duration, seed=p.seed,
                                threads=threads, use_libm_exp=1)
     _, _, secs = g.simulate(op)
-    return {"value": w.n_nodes * M_s * T_s / secs, "unit": "replica*node*steps/s",
+    return {"value": n_nodes * M_s * T_s / secs, "unit": "replica*node*steps/s",
             "cores": threads, "kind": "port",
-            "sample": f"{w.name}: N={w.n_nodes}, M={M_s} of {p.M}, first {T_s} of "
+            "sample": f"{name}: {note}N={n_nodes}, M={M_s} of {p.M}, first {T_s} of "
                       f"{p.iterations} iterations; loop {secs:.2f} s (glibc expf, OpenMP rows "
                       f"when nnz > 20000)"}
+
+
+def roofline(info: dict, per_launch_s: float) -> dict:
+    b = float(info["step_bytes"])
+    achieved = b / per_launch_s / 1e9
+    return {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": "step_kernel",
+            "bytes_per_launch": b, "us_per_launch": per_launch_s * 1e6,
+            "bytes_definition": "per 64-replica word: susceptible word read+write, own "
+                                "infectious word read, next infectious word write, infection "
+                                "word write, one ring read per timer offset; plus the slice "
+                                "epoch's 16-byte contact entries and row pointers"}
+
+
+def time_runs(sim, T: int, steps: int, warmup: int):
+    for _ in range(warmup):
+        sim.reset()
+        sim.run()
+    sim.sync()
+    kernel_ms = []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        sim.reset()
+        kernel_ms.append(sim.run_timed(T))
+    sim.sync()
+    return time.perf_counter() - t0, kernel_ms
+
+
+def bench_single(args, w) -> dict:
+    from replay_amd import Simulation
+    t0 = time.time()
+    sim = Simulation.from_contacts(w.n_nodes, w.src, w.tgt, w.start, w.end, w.params)
+    info = sim.info()
+    N, M, T = info["n_nodes"], info["M"], w.params.iterations
+    log(f"{w.name}: N={N} M={M} T={T} contacts={info['n_contacts']} windows={info['n_windows']} "
+        f"epochs={info['n_epochs']} entries={info['n_entries']} open {time.time() - t0:.1f}s")
+    wall, kernel_ms = time_runs(sim, T, args.steps, args.warmup)
+    counts = sim.counts()
+    sim.close()
+    per_launch = (sum(kernel_ms) / len(kernel_ms)) / 1e3 / T
+    out = {
+        "metric": "replica*node*steps/s (plain step loop, one network)",
+        "value": N * M * T * args.steps / wall,
+        "ms_per_step": wall * 1e3 / args.steps,
+        "config": {"workload": w.name, "nodes": N, "replicas": M, "iterations": T,
+                   "contacts": info["n_contacts"], "step_s": w.params.step_size,
+                   "window_s": w.params.static_network_duration,
+                   "windows_per_epoch": info["windows_per_epoch"], "variant": info["variant"],
+                   "final_recovered_mean": float(counts[-1, :, 2].mean())},
+        "roofline": roofline(info, per_launch),
+    }
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(w.name, w.n_nodes, w.src, w.tgt, w.start, w.end,
+                                           w.params, min(16, os.cpu_count() or 1), 64, 240)
+    return out
+
+
+def bench_batched(args, w) -> dict:
+    import numpy as np
+    import replay_amd
+    from replay_amd import Params, Simulation
+    sp = w.params
+    T, M = sp.iterations, sp.M
+    t0 = time.time()
+    bp = Params(**{**sp.__dict__, "flags": replay_amd.F_GRAPH})
+    batch = Simulation.from_networks(w.networks, bp)
+    variant = batch.autotune(min(T, 24))
+    info = batch.info()
+    N = info["n_nodes"]
+    log(f"{w.name}: {len(w.networks)} networks, N={N} M={M} T={T} contacts={info['n_contacts']} "
+        f"epochs={info['n_epochs']} entries={info['n_entries']} variant={variant} "
+        f"open {time.time() - t0:.1f}s")
+    wall, kernel_ms = time_runs(batch, T, args.steps, args.warmup)
+    bcounts = batch.counts()
+    per_launch = (sum(kernel_ms) / len(kernel_ms)) / 1e3 / T
+    batched_value = N * M * T * args.steps / wall
+    log(f"batched: {wall * 1e3 / args.steps:.2f} ms per run, {batched_value:.3e} r*n*s/s")
+    batch.close()
+
+    # plain per-network launch loop: one context per network, eager launches
+    t0 = time.time()
+
+    def open_plain(nw):
+        pp = Params(M=M, step_size=sp.step_size, iterations=T,
+                    initial_infected=nw.initial_infected, infect_prob=nw.infect_prob,
+                    exposed_duration=nw.exposed_duration,
+                    infectious_duration=nw.infectious_duration,
+                    resistant_duration=nw.resistant_duration, start_time=sp.start_time,
+                    static_network_duration=sp.static_network_duration, seed=sp.seed,
+                    stream=nw.stream, windows_per_epoch=info["windows_per_epoch"])
+        return Simulation.from_contacts(nw.n_nodes, nw.src, nw.tgt, nw.start, nw.end, pp)
+
+    from concurrent.futures import ThreadPoolExecutor
+    with ThreadPoolExecutor(max_workers=8) as ex:
+        plains = list(ex.map(open_plain, w.networks))
+    log(f"plain contexts open {time.time() - t0:.1f}s")
+    for s in plains:  # warmup
+        s.reset()
+        s.run()
+    for s in plains:
+        s.sync()
+    plain_steps = max(1, min(args.steps, 2))
+    t0 = time.perf_counter()
+    for _ in range(plain_steps):
+        for s in plains:  # one network at a time, as the reference runs them
+            s.reset()
+            s.run()
+            s.sync()
+    plain_wall = time.perf_counter() - t0
+    plain_value = N * M * T * plain_steps / plain_wall
+    identical = all(np.array_equal(s.counts(), bcounts[:, k]) for k, s in enumerate(plains))
+    for s in plains:
+        s.close()
+    log(f"plain: {plain_wall * 1e3 / plain_steps:.2f} ms per run, {plain_value:.3e} r*n*s/s; "
+        f"identical counts: {identical}")
+    out = {
+        "metric": "replica*node*steps/s (batched persistent form; plain per-network loop beside)",
+        "value": batched_value,
+        "ms_per_step": wall * 1e3 / args.steps,
+        "config": {"workload": w.name, "networks": len(w.networks), "nodes": N, "replicas": M,
+                   "iterations": T, "contacts": info["n_contacts"], "step_s": sp.step_size,
+                   "windows_per_epoch": info["windows_per_epoch"], "variant": variant,
+                   "hipgraph": True,
+                   "plain_value": plain_value, "plain_ms_per_step": plain_wall * 1e3 / plain_steps,
+                   "batched_over_plain": batched_value / plain_value,
+                   "batched_equals_plain": identical,
+                   "final_recovered_mean": float(bcounts[-1, :, :, 2].mean())},
+        "roofline": roofline(info, per_launch),
+    }
+    if not args.no_cpu_baseline:
+        nw = w.networks[0]
+        p0 = Params(M=M, step_size=sp.step_size, iterations=T,
+                    initial_infected=nw.initial_infected, infect_prob=nw.infect_prob,
+                    exposed_duration=nw.exposed_duration,
+                    infectious_duration=nw.infectious_duration,
+                    resistant_duration=nw.resistant_duration, start_time=sp.start_time,
+                    static_network_duration=sp.static_network_duration, seed=sp.seed)
+        out["cpu_baseline"] = cpu_baseline(w.name, nw.n_nodes, nw.src, nw.tgt, nw.start, nw.end,
+                                           p0, min(16, os.cpu_count() or 1), 32, 48,
+                                           note="network 0 of 64, ")
+    return out
 
 
 def main() -> int:
@@ -66,7 +208,7 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="config1")
+    ap.add_argument("--workload", default="config3")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -74,65 +216,21 @@ def main() -> int:
               "are independent, so the path shards with no per-step exchange; multi-GPU runs "
               "are not part of this build.", file=sys.stderr)
         return 2
-
-    from replay_amd import Simulation, workloads
+    from replay_amd import workloads
     t0 = time.time()
-    w = workloads.CONFIGS[args.workload]()
-    gen_s = time.time() - t0
-    t0 = time.time()
-    sim = Simulation.from_contacts(w.n_nodes, w.src, w.tgt, w.start, w.end, w.params)
-    open_s = time.time() - t0
-    info = sim.info()
-    N, M, T = info["n_nodes"], info["M"], w.params.iterations
-    print(f"[bench] {w.name}: N={N} M={M} T={T} contacts={info['n_contacts']} "
-          f"windows={info['n_windows']} epochs={info['n_epochs']} entries={info['n_entries']} "
-          f"gen {gen_s:.1f}s open {open_s:.1f}s", file=sys.stderr, flush=True)
-
-    for _ in range(args.warmup):
-        sim.reset()
-        sim.run()
-    sim.sync()
-    kernel_ms = []
-    t_start = time.perf_counter()
-    for _ in range(args.steps):
-        sim.reset()
-        kernel_ms.append(sim.run_timed(T))
-    sim.sync()
-    wall = time.perf_counter() - t_start
-    counts = sim.counts()
-    sim.close()
-
-    value = N * M * T * args.steps / wall
-    per_launch_s = (sum(kernel_ms) / len(kernel_ms)) / 1e3 / T
-    bytes_launch = step_kernel_bytes(info)
-    achieved = bytes_launch / per_launch_s / 1e9
-    out = {
-        "metric": "replica*node*steps/s (plain fused step loop, one network)",
-        "value": value,
-        "unit": "replica*node*steps/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": wall * 1e3 / args.steps,
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f32",
-        "data": "synthetic",
-        "config": {"workload": w.name, "nodes": N, "replicas": M, "iterations": T,
-                   "contacts": info["n_contacts"], "step_s": w.params.step_size,
-                   "window_s": w.params.static_network_duration,
-                   "windows_per_epoch": info["windows_per_epoch"],
-                   "final_recovered_mean": float(counts[-1, :, 2].mean())},
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "step_kernel", "bytes_per_launch": bytes_launch,
-                     "us_per_launch": per_launch_s * 1e6},
-        "cpu_baseline": None,
-    }
-    if not args.no_cpu_baseline:
-        threads = min(16, os.cpu_count() or 1)
-        out["cpu_baseline"] = cpu_baseline(w, threads)
+    if args.workload == "config3":
+        w = workloads.config3_batched()
+        log(f"generated {w.name} ({w.n_contacts} contacts) in {time.time() - t0:.1f}s")
+        res = bench_batched(args, w)
+    else:
+        w = workloads.CONFIGS[args.workload]()
+        log(f"generated {w.name} ({w.n_contacts} contacts) in {time.time() - t0:.1f}s")
+        res = bench_single(args, w)
+    out = {"metric": res["metric"], "value": res["value"], "unit": "replica*node*steps/s",
+           "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+           "ms_per_step": res["ms_per_step"], "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": None, "dtype": "f32", "data": "synthetic", "config": res["config"],
+           "roofline": res["roofline"], "cpu_baseline": res.get("cpu_baseline")}
     print(json.dumps(out))
     return 0
 

@@ -60,11 +60,35 @@ typedef struct replay_params {
   uint64_t seed;              /* Philox4x32-10 key (new: the reference seeds by clock) */
   int32_t device;             /* HIP device ordinal */
   int32_t windows_per_epoch;  /* slice packing; <= 0 picks automatically */
-  int64_t max_slice_bytes;    /* device budget for packed slices; <= 0: 32 GiB */
+  int64_t max_slice_bytes;    /* device budget for packed slices; <= 0: 8 GiB */
+  uint32_t stream;            /* Philox stream id of this network (counter word 3) */
+  int32_t flags;              /* REPLAY_F_* */
 } replay_params;
 
+/* replay_params.flags */
+#define REPLAY_F_GRAPH 1      /* full runs from iteration 0 replay one captured hipGraph */
+#define REPLAY_F_AUTOTUNE 2   /* time the kernel variants at open and keep the fastest */
+
+/* One network of a batch (replay_open_batch): its contacts, as in
+   replay_open_contacts, and its own epidemiological parameters. */
+typedef struct replay_network {
+  int32_t n_node_list;
+  const int64_t *node_ids;    /* NULL: ids 0..n_node_list-1 */
+  int64_t n_contacts;
+  const int32_t *src;
+  const int32_t *tgt;
+  const int64_t *start;
+  const int64_t *end;
+  float initial_infected;
+  float infect_prob;
+  int32_t exposed_duration;
+  int32_t infectious_duration;
+  int32_t resistant_duration;
+  uint32_t stream;            /* Philox stream id; equal ids give equal draws */
+} replay_network;
+
 typedef struct replay_info {
-  int32_t n_nodes;            /* N (NODE_LIST count) */
+  int32_t n_nodes;            /* N (NODE_LIST count; all networks of a batch) */
   int32_t M;
   int64_t n_contacts;         /* valid contacts (intervals) */
   int64_t nlists;             /* hit cap of every window query (cu:988-989) */
@@ -79,8 +103,15 @@ typedef struct replay_info {
   int32_t truncated_windows;  /* windows whose query hits the cap */
   int64_t step_bytes;         /* algorithmic HBM bytes of one step-kernel launch */
   int32_t ring_depth;         /* iterations of infection bit words kept on device */
-  int32_t reserved;
+  int32_t n_networks;         /* networks in the context (1 unless batched) */
+  int32_t variant;            /* step-kernel variant in use (REPLAY_V_*) */
+  int32_t graph_ready;        /* a captured full-run hipGraph exists */
 } replay_info;
+
+/* replay_info.variant */
+#define REPLAY_V_WORD 0       /* one pass over a node's contacts per 64-replica word */
+#define REPLAY_V_NODE4 4      /* one pass per 4 words, accumulated in registers */
+#define REPLAY_V_NODE16 16    /* one pass per 16 words, accumulated in registers */
 
 typedef struct replay_ctx replay_ctx;
 
@@ -108,6 +139,30 @@ int32_t replay_open_contacts(int32_t n_node_list, const int64_t *node_ids,
                              const int64_t *end, const replay_params *p,
                              replay_ctx **out);
 
+/* Several networks in one context: one step launch advances all of them
+   (the batched form; the reference runs one network per process).  Nodes
+   of network n are rows node_off[n] .. node_off[n+1]-1 of the context; the
+   shared parameters (M, step, iterations, windows, start time, seed, device,
+   flags) come from p, the epidemiological ones from each replay_network.
+   A network gets the same draws as in a single-network context with
+   p->stream equal to its stream id, so both forms give identical results. */
+int32_t replay_open_batch(int32_t n_networks, const replay_network *nets,
+                          const replay_params *p, replay_ctx **out);
+
+/* node_off[0..n_networks]: first row of each network, then the total. */
+int32_t replay_get_network_offsets(replay_ctx *ctx, int32_t *node_off,
+                                   int32_t n);
+
+/* Times `steps` iterations of each applicable step-kernel variant (from a
+   reset state, resetting again afterwards), keeps the fastest and returns
+   its REPLAY_V_* code in *variant.  No counterpart in the reference: the
+   tile/schedule choice cuSPARSE makes internally (cu:1039-1048). */
+int32_t replay_autotune(replay_ctx *ctx, int32_t steps, int32_t *variant);
+
+/* Forces a step-kernel variant (REPLAY_V_*); REPLAY_E_ARG if it does not
+   apply to this context. */
+int32_t replay_set_variant(replay_ctx *ctx, int32_t variant);
+
 /* Sizes and counters of an open context. */
 int32_t replay_get_info(replay_ctx *ctx, replay_info *out);
 
@@ -130,11 +185,13 @@ int32_t replay_sync(replay_ctx *ctx);
 
 /* Per-iteration SEIR counts in CSV column order (exposed, infectious,
    resistant, susceptible) -- the reduction write_simulation_state performs
-   (cu:505-568).  out holds (steps_done + 1) * M * 4 int32; block 0 is the
-   initial state.  Synchronises. */
+   (cu:505-568).  out holds (steps_done + 1) * n_networks * M * 4 int32,
+   ordered [iteration][network][sim][class]; block 0 is the initial state.
+   Synchronises. */
 int32_t replay_get_counts(replay_ctx *ctx, int32_t *out, int64_t n_out);
 
-/* Countdown state, N x M int32 column-major (cu:1087-1088).  Synchronises. */
+/* Countdown state, N x M int32 column-major (cu:1087-1088); for a batch,
+   N counts the rows of all networks.  Synchronises. */
 int32_t replay_get_state(replay_ctx *ctx, int32_t *out, int64_t n_out);
 
 /* Timestamp written with iteration k's block (cu:876, 1093-1095); k = -1

@@ -757,7 +757,7 @@ uint32_t or_philox_word(uint64_t seed, uint32_t c0, uint32_t c1, uint32_t c2,
 }
 
 /* Uniform for (slot, node, replica): slot 0 = initial draw, slot k+1 = draw
-   of iteration k.  Counter = {node, replica, slot, 0}, key = seed; the top
+   of iteration k.  Counter = {node, replica, slot, stream}, key = seed; the top
    24 bits of the first output word scaled by 2^-24, so u is in [0, 1).
    An injected stream is indexed [slot][replica][node]. */
 float or_uniform(const or_params *p, int slot, int node, int replica, int n_nodes) {
@@ -765,7 +765,7 @@ float or_uniform(const or_params *p, int slot, int node, int replica, int n_node
     return p->uniforms[((size_t)slot * (size_t)p->M + (size_t)replica) * (size_t)n_nodes +
                        (size_t)node];
   uint32_t w = or_philox_word(p->seed, (uint32_t)node, (uint32_t)replica,
-                              (uint32_t)slot, 0u);
+                              (uint32_t)slot, p->stream);
   return (float)(w >> 8) * 0x1p-24f;
 }
 

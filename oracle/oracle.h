@@ -74,6 +74,7 @@ typedef struct {
   int32_t threads;      /* >1 enables OpenMP rows when nnz > 20000 */
   int32_t use_libm_exp; /* 1: glibc expf (reference-faithful timing) */
   const float *uniforms;/* optional injected stream, see or_uniform() */
+  uint32_t stream;      /* Philox counter word 3 */
 } or_params;
 
 enum {

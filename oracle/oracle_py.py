@@ -45,7 +45,7 @@ class Params(C.Structure):
                 ("resistant_duration", C.c_int32), ("start_time", C.c_int64),
                 ("static_network_duration", C.c_int64), ("seed", C.c_uint64),
                 ("threads", C.c_int32), ("use_libm_exp", C.c_int32),
-                ("uniforms", C.POINTER(C.c_float))]
+                ("uniforms", C.POINTER(C.c_float)), ("stream", C.c_uint32)]
 
 
 _lib = None
@@ -112,10 +112,10 @@ def make_params(M=4, step_size=3600.0, iterations=10, initial_infected=0.0,
                 infect_prob=0.0, exposed_duration=0, infectious_duration=0,
                 resistant_duration=0, start_time=946713600,
                 static_network_duration=3600, seed=0, threads=1,
-                use_libm_exp=0, uniforms: np.ndarray | None = None) -> Params:
+                use_libm_exp=0, uniforms: np.ndarray | None = None, stream=0) -> Params:
     p = Params(M, step_size, iterations, initial_infected, infect_prob, exposed_duration,
                infectious_duration, resistant_duration, start_time,
-               static_network_duration, seed, threads, use_libm_exp, None)
+               static_network_duration, seed, threads, use_libm_exp, None, stream)
     if uniforms is not None:
         p._uniforms_keepalive = uniforms
         p.uniforms = _ptr(uniforms, C.c_float)

@@ -42,7 +42,17 @@ class _Params(C.Structure):
                 ("resistant_duration", C.c_int32), ("start_time", C.c_int64),
                 ("static_network_duration", C.c_int64), ("seed", C.c_uint64),
                 ("device", C.c_int32), ("windows_per_epoch", C.c_int32),
-                ("max_slice_bytes", C.c_int64)]
+                ("max_slice_bytes", C.c_int64), ("stream", C.c_uint32), ("flags", C.c_int32)]
+
+
+class _Network(C.Structure):
+    _fields_ = [("n_node_list", C.c_int32), ("node_ids", C.POINTER(C.c_int64)),
+                ("n_contacts", C.c_int64), ("src", C.POINTER(C.c_int32)),
+                ("tgt", C.POINTER(C.c_int32)), ("start", C.POINTER(C.c_int64)),
+                ("end", C.POINTER(C.c_int64)), ("initial_infected", C.c_float),
+                ("infect_prob", C.c_float), ("exposed_duration", C.c_int32),
+                ("infectious_duration", C.c_int32), ("resistant_duration", C.c_int32),
+                ("stream", C.c_uint32)]
 
 
 class _Info(C.Structure):
@@ -52,7 +62,13 @@ class _Info(C.Structure):
                 ("n_entries", C.c_int64), ("slice_bytes", C.c_int64),
                 ("state_bytes", C.c_int64), ("steps_done", C.c_int32),
                 ("truncated_windows", C.c_int32), ("step_bytes", C.c_int64),
-                ("ring_depth", C.c_int32), ("reserved", C.c_int32)]
+                ("ring_depth", C.c_int32), ("n_networks", C.c_int32), ("variant", C.c_int32),
+                ("graph_ready", C.c_int32)]
+
+
+F_GRAPH = 1
+F_AUTOTUNE = 2
+V_WORD, V_NODE4, V_NODE16 = 0, 4, 16
 
 
 EXPORTED_SYMBOLS = (
@@ -61,6 +77,7 @@ EXPORTED_SYMBOLS = (
     "replay_get_counts",
     "replay_get_state", "replay_iteration_time", "replay_set_uniforms", "replay_get_slice",
     "replay_get_exposure", "replay_last_error", "replay_error_string", "replay_close",
+    "replay_open_batch", "replay_get_network_offsets", "replay_autotune", "replay_set_variant",
 )
 
 _lib = None
@@ -87,6 +104,10 @@ def load_library() -> C.CDLL:
     L.replay_open_contacts.argtypes = [C.c_int32, P(C.c_int64), C.c_int64, P(C.c_int32),
                                        P(C.c_int32), P(C.c_int64), P(C.c_int64), P(_Params),
                                        P(ctx_p)]
+    L.replay_open_batch.argtypes = [C.c_int32, P(_Network), P(_Params), P(ctx_p)]
+    L.replay_get_network_offsets.argtypes = [ctx_p, P(C.c_int32), C.c_int32]
+    L.replay_autotune.argtypes = [ctx_p, C.c_int32, P(C.c_int32)]
+    L.replay_set_variant.argtypes = [ctx_p, C.c_int32]
     L.replay_get_info.argtypes = [ctx_p, P(_Info)]
     L.replay_reset.argtypes = [ctx_p]
     L.replay_run.argtypes = [ctx_p, C.c_int32]
@@ -108,7 +129,9 @@ def load_library() -> C.CDLL:
     L.replay_close.restype = None
     for name in ("replay_open", "replay_open_contacts", "replay_get_info", "replay_reset",
                  "replay_run", "replay_run_timed", "replay_sync", "replay_get_counts", "replay_get_state",
-                 "replay_set_uniforms", "replay_get_slice", "replay_get_exposure"):
+                 "replay_set_uniforms", "replay_get_slice", "replay_get_exposure",
+                 "replay_open_batch", "replay_get_network_offsets", "replay_autotune",
+                 "replay_set_variant"):
         getattr(L, name).restype = C.c_int32
     if L.replay_abi_version() != ABI_VERSION:
         raise RuntimeError("libreplay_hip ABI version mismatch")
@@ -134,12 +157,33 @@ class Params:
     device: int = 0
     windows_per_epoch: int = 0
     max_slice_bytes: int = 0
+    stream: int = 0
+    flags: int = 0
 
     def _c(self) -> _Params:
         return _Params(self.M, self.step_size, self.iterations, self.initial_infected,
                        self.infect_prob, self.exposed_duration, self.infectious_duration,
                        self.resistant_duration, self.start_time, self.static_network_duration,
-                       self.seed, self.device, self.windows_per_epoch, self.max_slice_bytes)
+                       self.seed, self.device, self.windows_per_epoch, self.max_slice_bytes,
+                       self.stream, self.flags)
+
+
+@dataclass
+class Network:
+    """One network of a batch: contacts (ids, Unix start/end) and its own
+    epidemiological parameters (replay_network)."""
+
+    n_nodes: int
+    src: np.ndarray
+    tgt: np.ndarray
+    start: np.ndarray
+    end: np.ndarray
+    initial_infected: float = 0.0
+    infect_prob: float = 0.0
+    exposed_duration: int = 0
+    infectious_duration: int = 0
+    resistant_duration: int = 0
+    stream: int = 0
 
 
 def _ptr(a: np.ndarray, ctype):
@@ -155,6 +199,10 @@ class Simulation:
         info = self.info()
         self.n_nodes = info["n_nodes"]
         self.M = info["M"]
+        self.n_networks = info["n_networks"]
+        off = np.zeros(self.n_networks + 1, np.int32)
+        self._check(load_library().replay_get_network_offsets(ctx, _ptr(off, C.c_int32), off.size))
+        self.node_offsets = off
 
     # construction -------------------------------------------------------
     @classmethod
@@ -182,6 +230,27 @@ class Simulation:
                                     len(src), _ptr(src, C.c_int32), _ptr(tgt, C.c_int32),
                                     _ptr(start, C.c_int64), _ptr(end, C.c_int64),
                                     C.byref(p), C.byref(ctx))
+        if rc != 0:
+            raise ReplayError(rc, L.replay_last_error(None).decode())
+        return cls(ctx, params)
+
+    @classmethod
+    def from_networks(cls, networks: list, params: Params) -> "Simulation":
+        """Batched form: every network advances in the same step launches."""
+        L = load_library()
+        keep = []
+        arr = (_Network * len(networks))()
+        for k, nw in enumerate(networks):
+            a = [np.ascontiguousarray(nw.src, np.int32), np.ascontiguousarray(nw.tgt, np.int32),
+                 np.ascontiguousarray(nw.start, np.int64), np.ascontiguousarray(nw.end, np.int64)]
+            keep.append(a)
+            arr[k] = _Network(nw.n_nodes, None, len(a[0]), _ptr(a[0], C.c_int32),
+                              _ptr(a[1], C.c_int32), _ptr(a[2], C.c_int64), _ptr(a[3], C.c_int64),
+                              nw.initial_infected, nw.infect_prob, nw.exposed_duration,
+                              nw.infectious_duration, nw.resistant_duration, nw.stream)
+        ctx = C.c_void_p()
+        p = params._c()
+        rc = L.replay_open_batch(len(networks), arr, C.byref(p), C.byref(ctx))
         if rc != 0:
             raise ReplayError(rc, L.replay_last_error(None).decode())
         return cls(ctx, params)
@@ -238,11 +307,21 @@ class Simulation:
         return self.info()["steps_done"]
 
     def counts(self) -> np.ndarray:
-        """(steps_done+1, M, 4) int32: exposed, infectious, resistant, susceptible."""
+        """(steps_done+1, M, 4) int32 for one network, (steps_done+1,
+        n_networks, M, 4) for a batch: exposed, infectious, resistant,
+        susceptible."""
         k = self.steps_done
-        out = np.zeros((k + 1, self.M, 4), np.int32)
+        out = np.zeros((k + 1, self.n_networks, self.M, 4), np.int32)
         self._check(load_library().replay_get_counts(self._ctx, _ptr(out, C.c_int32), out.size))
-        return out
+        return out[:, 0] if self.n_networks == 1 else out
+
+    def autotune(self, steps: int = 16) -> int:
+        v = C.c_int32(0)
+        self._check(load_library().replay_autotune(self._ctx, steps, C.byref(v)))
+        return v.value
+
+    def set_variant(self, variant: int) -> None:
+        self._check(load_library().replay_set_variant(self._ctx, variant))
 
     def state(self) -> np.ndarray:
         """N*M int32 countdowns, column-major (idx = node + sim*N)."""
@@ -295,5 +374,5 @@ def write_summary_csv(path: str, sim: Simulation) -> None:
                 f.write(f"{t},{s},{c[0]},{c[1]},{c[2]},{c[3]}\n")
 
 
-__all__ = ["Params", "Simulation", "ReplayError", "load_library", "library_path",
+__all__ = ["Params", "Network", "Simulation", "ReplayError", "load_library", "library_path",
            "write_summary_csv", "EXPORTED_SYMBOLS", "CLI_PATH", "LIB_PATH"]

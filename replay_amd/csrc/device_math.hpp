@@ -34,10 +34,10 @@ __device__ __forceinline__ uint32_t philox_word0(uint64_t seed, uint32_t c0, uin
   return c0;
 }
 
-// Uniform in [0, 1): top 24 bits of Philox(counter = {node, sim, slot, 0}).
+// Uniform in [0, 1): top 24 bits of Philox(counter = {node, sim, slot, stream}).
 __device__ __forceinline__ float philox_uniform(uint64_t seed, uint32_t node, uint32_t sim,
-                                                uint32_t slot) {
-  return static_cast<float>(philox_word0(seed, node, sim, slot, 0u) >> 8) * 0x1p-24f;
+                                                uint32_t slot, uint32_t stream) {
+  return static_cast<float>(philox_word0(seed, node, sim, slot, stream) >> 8) * 0x1p-24f;
 }
 
 __device__ __forceinline__ float rp_expf(float x) {

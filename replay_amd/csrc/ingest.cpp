@@ -462,6 +462,34 @@ struct Span {
 
 }  // namespace
 
+std::vector<int64_t> slice_bytes(const Network &net, const Plan &plan,
+                                 const std::vector<int64_t> &candidates) {
+  const auto &cs = net.contacts;
+  const int32_t W = static_cast<int32_t>(plan.windows.size());
+  const int64_t N = net.n_nodes;
+  std::vector<int64_t> out(candidates.size(), 0);
+  if (W == 0) return out;
+  std::vector<int64_t> ws(W), we(W);
+  for (int32_t k = 0; k < W; ++k) {
+    ws[k] = plan.windows[k].ws;
+    we[k] = plan.windows[k].we;
+  }
+  std::vector<int64_t> ne(candidates.size(), 0);
+  for (const Contact &c : cs) {
+    if (c.end <= c.start) continue;
+    const int64_t a = std::upper_bound(we.begin(), we.end(), c.start) - we.begin();
+    const int64_t b = (std::lower_bound(ws.begin(), ws.end(), c.end) - ws.begin()) - 1;
+    if (a > b) continue;
+    for (size_t j = 0; j < candidates.size(); ++j)
+      ne[j] += 2 * (b / candidates[j] - a / candidates[j] + 1);
+  }
+  for (size_t j = 0; j < candidates.size(); ++j) {
+    const int64_t S = candidates[j];
+    out[j] = ne[j] * static_cast<int64_t>(sizeof(Entry)) + ((W + S - 1) / S) * (N + 1) * 4;
+  }
+  return out;
+}
+
 int build_slices(const Network &net, Plan &plan, int32_t windows_per_epoch,
                  int64_t max_bytes, Slices &out, std::string &err) {
   const auto &cs = net.contacts;

@@ -79,5 +79,8 @@ int plan_windows(const Network &net, const replay_params &p, Plan &plan,
 // max_bytes (or the smallest per-step footprint when everything fits).
 int build_slices(const Network &net, Plan &plan, int32_t windows_per_epoch,
                  int64_t max_bytes, Slices &out, std::string &err);
+// Packed bytes of the slices for each candidate windows-per-epoch value.
+std::vector<int64_t> slice_bytes(const Network &net, const Plan &plan,
+                                 const std::vector<int64_t> &candidates);
 
 }  // namespace rp

@@ -1,45 +1,51 @@
 // replay_hip.hip -- libreplay_hip: HIP kernels for gfx950 and the C ABI
 // declared in include/replay_hip.h.
 //
-// Replica state is bit-packed.  A word (node i, q) holds 64 replicas
-// 64q .. 64q+63; per word the device keeps
+// A context holds one or more networks (a batch) as a disjoint union of node
+// rows; one step launch advances every network.  Replica state is
+// bit-packed: a word (node i, q) holds replicas 64q .. 64q+63, and per word
+// the device keeps
 //   S       susceptible bits at the current iteration     (countdown == 0)
 //   I[2]    infectious bits, double-buffered               (lower < c <= medium)
 //   X ring  infection bits of each past iteration, depth D
 //   X_init  bits seeded infected by the initial draw
 // plus, per replica, entry[i][r] (iteration of the last infection, written
-// only when one happens) and inf_count[k][r] (infections of iteration k).
+// only when one happens) and inf_count[k][net][r] (infections of iteration k).
 //
 // Why this is exact: after an infection the countdown follows one fixed
 // sequence, c(n) = f^n(c0) with f(c) = (int)((float)c - step) clamped at 0
 // (tool/gpu_cpu_temporal_sim.cu:153-175), the same for every node and
-// replica of that kind (new infection: c0 = (int)(upper + step); initial
-// seed: c0 = (int)(medium + step), cu:85).  The host tabulates c(n) once,
-// with the same float operations, and derives the offsets at which a
-// replica turns infectious (n_on), stops being infectious (n_off) and
-// becomes susceptible again (n_S).  So
+// replica of a network (new infection: c0 = (int)(upper + step); initial
+// seed: c0 = (int)(medium + step), cu:85).  The host tabulates c(n) once per
+// network with the same float operations and derives the offsets at which a
+// replica turns infectious (n_on), stops being infectious (n_off) and becomes
+// susceptible again (n_S).  So
 //   I(k+1) = I(k) ^ X(k+1-n_on) ^ X(k+1-n_off),  S(k+1) = S(k) & ~X(k) | X(k+1-n_S)
 // and the exposed/infectious/resistant counts of iteration k are sums of
 // inf_count over the offset ranges where the tabulated class
 // (cu:530-550) is E, I or R.
 //
-// One step kernel per iteration: a block owns 256 consecutive words.  Every
-// thread loads its word's S; words of nodes that have contacts in the current
-// slice and still hold susceptible replicas are queued in LDS, and the
-// block's wavefronts take them one at a time with one lane per replica:
-// each walks the node's packed contact entries in ascending column order,
-// adds each contact's overlap with the window where the neighbour's
-// infectious bit is set, converts exposure to 1 - exp(log(1-p) * E/3600),
-// draws a Philox uniform and ballots the infections.  Then every thread
-// applies the flips of its word.
+// Step kernel: a block owns 256 consecutive words.  Every thread loads its
+// word's S; words of nodes that have contacts in the current slice and still
+// hold susceptible replicas are queued in LDS, and the block's wavefronts
+// take the queue one item at a time with one lane per replica: each walks the
+// node's packed contact entries in ascending column order, adds each
+// contact's overlap with the window where the neighbour's infectious bit is
+// set, converts exposure to 1 - exp(log(1-p) * E/3600), draws a Philox
+// uniform and ballots the infections.  Then every thread applies its word's
+// flips.  Variant NODE4/NODE16 queues one item per node and keeps all of the
+// node's words in registers (integer exposure only); WORD queues one item
+// per word.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <climits>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "device_math.hpp"
@@ -61,35 +67,46 @@ struct Segs {  // runs of the tabulated class over n = 1..T for new infections
   int32_t cls[kMaxSegs];  // 1 E, 2 I, 3 R
 };
 
+// Constants of one network on the device.
+struct NetDev {
+  float log_base;    // logf(1 - infect_prob), computed on the host
+  float p0;          // initial_infected
+  int32_t node_off;  // first row
+  uint32_t stream;   // Philox counter word 3
+  int32_t n_on, n_off, n_s;  // flip offsets of a new infection (kNever: none)
+  int32_t m_on, m_off, m_s;  // the same for the initial seeds
+};
+
 struct StepArgs {
-  const int32_t *rowptr;   // slice epoch rowptr (N+1), epoch-relative entries
-  const int4 *entries;     // slice epoch entries {col, ts, te, rank}
-  const uint64_t *I_cur;   // infectious bits at iteration k
-  uint64_t *I_next;        // infectious bits at iteration k+1
-  uint64_t *S;             // susceptible bits, k -> k+1 in place
-  uint64_t *X_k;           // ring slot of iteration k
-  const uint64_t *X_on;    // X(k+1-n_on) or null
-  const uint64_t *X_off;   // X(k+1-n_off) or null
-  const uint64_t *X_s;     // X(k+1-n_S) or null
+  const int32_t *rowptr;      // slice epoch rowptr (N+1), epoch-relative entries
+  const int4 *entries;        // slice epoch entries {col, ts, te, rank}
+  const uint64_t *I_cur;      // infectious bits at iteration k
+  uint64_t *I_next;           // infectious bits at iteration k+1
+  uint64_t *S;                // susceptible bits, k -> k+1 in place
+  uint64_t *X;                // infection ring, depth x NW
   const uint64_t *X_init;
-  int32_t *entry;          // [N][Mp]
-  int32_t *inf_count;      // [T+1][M]
-  int32_t *P;              // prefix sums of inf_count, [T+2][M]
-  const float *uniforms;   // injected stream for this slot, [M][N], or null
+  int32_t *entry;             // [N][Mp]
+  int32_t *inf_count;         // [T+1][n_net][M]
+  int32_t *P;                 // prefix sums of inf_count, [T+2][n_net][M]
+  const float *uniforms;      // injected stream for this slot, [M][N], or null
+  const NetDev *nets;
+  const int32_t *net_of;      // node -> network
+  const uint32_t *rank_limit; // [n_net] of this window, or null (no cap bites)
   uint64_t seed;
-  int64_t NW;              // N * W words
-  int32_t N, M, W, Mp;
+  int64_t NW;                 // N * W words
+  int32_t N, M, W, Mp, n_net, depth;
   int32_t k;
   int32_t ws_rel, we_rel;
-  uint32_t rank_limit;
   float scale;
-  float log_base;          // logf(1 - infect_prob), computed on the host
-  int32_t c_new;           // countdown of a new infection, (int)(upper + step)
   uint32_t slot;
-  // flips of local x (n == 1) and of the initial seeds at iteration k+1
-  int32_t on_is_local, off_is_local, s_is_local;
-  int32_t init_flip_I, init_set_S;
 };
+
+__device__ __forceinline__ const uint64_t *ring_word(const StepArgs &a, int32_t n, int64_t w) {
+  // X(k+1-n) for n >= 2 (n == 1 is the local infection word)
+  const int32_t e = a.k + 1 - n;
+  if (n == kNever || n < 2 || e < 0) return nullptr;
+  return a.X + static_cast<size_t>(e % a.depth) * a.NW + w;
+}
 
 // Exposure seconds of (row i, replica lane) for one window.
 // Integer mode (divisor 1, row weight sums < 2^24): overlaps are integers and
@@ -98,11 +115,10 @@ struct StepArgs {
 // are merged in rank order (cu:299-309), and columns are added in ascending
 // order from zero (Eigen SparseDenseProduct.h:64-70).
 template <bool kIntMode>
-__device__ __forceinline__ float row_exposure(const int32_t *rowptr, const int4 *entries,
+__device__ __forceinline__ float row_exposure(int e0, int e1, const int4 *entries,
                                               const uint64_t *I, int32_t W, int32_t ws_rel,
                                               int32_t we_rel, uint32_t rank_limit, float scale,
-                                              int i, int q, int lane) {
-  const int e0 = rowptr[i], e1 = rowptr[i + 1];
+                                              int q, int lane) {
   if (kIntMode) {
     uint32_t acc = 0;
 #pragma unroll 4
@@ -140,166 +156,231 @@ __device__ __forceinline__ float row_exposure(const int32_t *rowptr, const int4 
   }
 }
 
-// Integer-mode exposure of the words q0 .. q0+nq-1 (nq <= WT) of node i at
-// once: each contact entry is read once and the neighbour's infectious words
-// for the whole range are fetched together, so a node costs a few memory
-// round trips instead of a few per word.
-template <int WT>
-__device__ __forceinline__ void node_exposure(int e0, int e1, const int4 *entries,
-                                              const uint64_t *I, int32_t W, int32_t ws_rel,
-                                              int32_t we_rel, uint32_t rank_limit, int q0,
-                                              int nq, int lane, uint32_t (&acc)[WT]) {
+// 64 x 64 bit-matrix transpose across a wavefront: lane j holds row j on
+// entry, lane r holds column r on exit (bit j of the result = bit r of row
+// j).  Six block-swap stages, largest blocks first: at stage s the rows
+// j (j & s == 0) and j + s exchange the s-bit column blocks that lie across
+// the diagonal of each 2s x 2s block.
+__device__ __forceinline__ uint64_t transpose64(uint64_t a, int lane) {
+  constexpr uint64_t kLow[6] = {0x00000000FFFFFFFFull, 0x0000FFFF0000FFFFull,
+                                0x00FF00FF00FF00FFull, 0x0F0F0F0F0F0F0F0Full,
+                                0x3333333333333333ull, 0x5555555555555555ull};
 #pragma unroll
-  for (int q = 0; q < WT; ++q) acc[q] = 0;
-  for (int e = e0; e < e1; ++e) {
-    const int4 en = entries[e];
-    const int ov = min(en.z, we_rel) - max(en.y, ws_rel);
-    if (ov > 0 && static_cast<uint32_t>(en.w) <= rank_limit) {
-      const uint64_t *mrow = I + static_cast<size_t>(en.x) * W + q0;
+  for (int k = 0; k < 6; ++k) {
+    const int s = 32 >> k;
+    const uint64_t lo = kLow[k];
+    const uint64_t b = __shfl_xor(a, s);
+    a = (lane & s) ? (((b & ~lo) >> s) | (a & ~lo)) : ((a & lo) | ((b & lo) << s));
+  }
+  return a;
+}
+
+// Exposure of the words q0 .. q0+nq-1 (nq <= WT) of node i for one window.
+// The wavefront loads 64 contact entries at a time (one per lane, coalesced)
+// together with the neighbours' infectious words of the range, then takes
+// the entries in order, broadcasting each one's overlap and words from its
+// lane: two memory round trips per 64 entries.  Integer mode adds overlaps
+// into exact integer sums; float mode merges the contributions fl(ov * scale)
+// of one column in rank order and adds columns in ascending order
+// (cu:299-309, Eigen SparseDenseProduct.h:64-70).
+template <bool kIntMode, int WT>
+__device__ __forceinline__ void group_exposure(const StepArgs &a, int e0, int e1,
+                                               uint32_t rank_limit, int q0, int nq, int lane,
+                                               float (&E)[WT]) {
+  uint32_t iacc[WT];
+  float facc[WT];
+#pragma unroll
+  for (int q = 0; q < WT; ++q) {
+    iacc[q] = 0;
+    facc[q] = 0.0f;
+  }
+  // float mode: the column being merged, its weight and its words
+  int cur_col = -1;
+  float cur_w = 0.0f;
+  bool cur_any = false;
+  uint32_t cur_lo[WT], cur_hi[WT];
+#pragma unroll
+  for (int q = 0; q < WT; ++q) cur_lo[q] = cur_hi[q] = 0;
+  for (int c0 = e0; c0 < e1; c0 += kWave) {
+    const int e = c0 + lane;
+    const bool ev = e < e1;
+    int col = 0, ov = 0;
+    if (ev) {
+      const int4 en = a.entries[e];
+      col = en.x;
+      ov = min(en.z, a.we_rel) - max(en.y, a.ws_rel);
+      if (!(ov > 0 && static_cast<uint32_t>(en.w) <= rank_limit)) ov = 0;
+    }
+    uint32_t mlo[WT], mhi[WT];
+    const uint64_t *mrow = a.I_cur + static_cast<size_t>(col) * a.W + q0;
+#pragma unroll
+    for (int q = 0; q < WT; ++q) {
+      const uint64_t m = (ev && ov > 0 && q < nq) ? mrow[q] : 0ull;
+      mlo[q] = static_cast<uint32_t>(m);
+      mhi[q] = static_cast<uint32_t>(m >> 32);
+    }
+    if (kIntMode) {
+      // Transpose the 64 x 64 bit block (row j = entry j's word) so lane r
+      // holds bit r of every entry: contacts spanning the whole window add
+      // L * popcount, partial overlaps are added bit by bit.
+      const int L = a.we_rel - a.ws_rel;
+      const uint64_t full = __ballot(ov == L);
+      const uint64_t part = __ballot(ov > 0 && ov != L);
 #pragma unroll
       for (int q = 0; q < WT; ++q) {
-        if (q < nq) {
-          const uint64_t m = mrow[q];
-          acc[q] += static_cast<uint32_t>((m >> lane) & 1ull) * static_cast<uint32_t>(ov);
+        const uint64_t m = (static_cast<uint64_t>(mhi[q]) << 32) | mlo[q];
+        if (q >= nq || __ballot(m != 0ull) == 0ull) continue;
+        const uint64_t t = transpose64(m, lane);
+        uint32_t add = static_cast<uint32_t>(L) * static_cast<uint32_t>(__popcll(t & full));
+        for (uint64_t pm = part & __ballot(m != 0ull); pm != 0ull; pm &= pm - 1) {
+          const int j = __builtin_ctzll(pm);
+          const uint32_t ovj = static_cast<uint32_t>(__builtin_amdgcn_readlane(ov, j));
+          add += static_cast<uint32_t>((t >> j) & 1ull) * ovj;
+        }
+        iacc[q] += add;
+      }
+      continue;
+    }
+    const int n_here = min(kWave, e1 - c0);
+    for (int j = 0; j < n_here; ++j) {
+      const int ovj = __builtin_amdgcn_readlane(ov, j);
+      {
+        const int colj = __builtin_amdgcn_readlane(col, j);
+        if (colj != cur_col) {  // flush the finished column
+          if (cur_any) {
+#pragma unroll
+            for (int q = 0; q < WT; ++q) {
+              const uint32_t word = lane < 32 ? cur_lo[q] : cur_hi[q];
+              if ((word >> (lane & 31)) & 1u) facc[q] += cur_w;
+            }
+          }
+          cur_col = colj;
+          cur_w = 0.0f;
+          cur_any = false;
+        }
+        if (ovj > 0) {
+          const float wc = static_cast<float>(ovj) * a.scale;
+          cur_w = cur_any ? cur_w + wc : wc;
+          cur_any = true;
+#pragma unroll
+          for (int q = 0; q < WT; ++q) {
+            cur_lo[q] = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(mlo[q]), j));
+            cur_hi[q] = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(mhi[q]), j));
+          }
         }
       }
     }
   }
+  if (!kIntMode && cur_any) {
+#pragma unroll
+    for (int q = 0; q < WT; ++q) {
+      const uint32_t word = lane < 32 ? cur_lo[q] : cur_hi[q];
+      if ((word >> (lane & 31)) & 1u) facc[q] += cur_w;
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < WT; ++q) E[q] = kIntMode ? static_cast<float>(iacc[q]) : facc[q];
 }
 
-// Draw of one word's susceptible lanes given their exposure; returns the
-// ballot of new infections and records them.
+// Draw of one word's susceptible lanes given their exposure; records the
+// infections and returns their ballot.
 template <bool kInjected>
-__device__ __forceinline__ uint64_t draw_word(const StepArgs &a, int ii, int qq, int lane,
-                                              uint64_t ss, float E) {
+__device__ __forceinline__ uint64_t draw_word(const StepArgs &a, const NetDev &nd, int net,
+                                              int ii, int qq, int lane, uint64_t ss, float E) {
   const int r = qq * kWave + lane;
   const bool sus = (ss >> lane) & 1ull;
   bool infect = false;
   // E == 0 gives P = 0 (or NaN), never above a uniform in [0, 1)
   if (sus && (kInjected || E > 0.0f)) {
-    const float P = 1.0f - rp::rp_expf(a.log_base * (E / 3600.0f));
+    const float P = 1.0f - rp::rp_expf(nd.log_base * (E / 3600.0f));
     const float u = kInjected ? a.uniforms[static_cast<size_t>(r) * a.N + ii]
-                              : rp::philox_uniform(a.seed, static_cast<uint32_t>(ii),
-                                                   static_cast<uint32_t>(r), a.slot);
+                              : rp::philox_uniform(a.seed, static_cast<uint32_t>(ii - nd.node_off),
+                                                   static_cast<uint32_t>(r), a.slot, nd.stream);
     infect = P > u;
   }
   if (infect) {
     a.entry[static_cast<size_t>(ii) * a.Mp + r] = 2 * a.k + 1;
-    atomicAdd(a.inf_count + static_cast<size_t>(a.k) * a.M + r, 1);
+    atomicAdd(a.inf_count + (static_cast<size_t>(a.k) * a.n_net + net) * a.M + r, 1);
   }
   return __ballot(infect);
 }
 
-// WT > 0: integer mode with node-level items (W <= WT); WT == 0: one item per
-// word (float mode or wide nodes).
+// Phase 1 of an iteration: one wavefront per node computes the exposure of
+// its susceptible replicas, draws, and writes the node's infection words
+// (zero words included) into ring slot k.
 template <bool kIntMode, bool kInjected, int WT>
-__global__ __launch_bounds__(kBlock) void step_kernel(StepArgs a) {
-  __shared__ uint64_t s_sus[kBlock];
-  __shared__ uint64_t s_x[kBlock];
-  __shared__ int32_t s_list[kBlock];
-  __shared__ uint8_t s_need[kBlock];
-  __shared__ int2 s_rp[kBlock];
-  __shared__ int32_t s_n;
-  const int t = threadIdx.x;
-  const int64_t w = static_cast<int64_t>(blockIdx.x) * kBlock + t;
-  const bool wv = w < a.NW;
-  const int i = wv ? static_cast<int>(w / a.W) : 0;
-  const uint64_t s = wv ? a.S[w] : 0ull;
-  // the word-local flips do not depend on this iteration's draws: issue
-  // their loads now so they overlap the exposure phase
-  uint64_t inf_pre = 0ull, sus_add = 0ull;
-  if (wv) {
-    inf_pre = a.I_cur[w];
-    if (a.X_on) inf_pre ^= a.X_on[w];
-    if (a.X_off) inf_pre ^= a.X_off[w];
-    if (a.X_s) sus_add = a.X_s[w];
-    if (a.init_flip_I || a.init_set_S) {
-      const uint64_t xi = a.X_init[w];
-      if (a.init_flip_I & 1) inf_pre ^= xi;
-      if (a.init_flip_I & 2) inf_pre ^= xi;
-      if (a.init_set_S) sus_add |= xi;
-    }
-  }
-  const int32_t rp0 = wv ? a.rowptr[i] : 0, rp1 = wv ? a.rowptr[i + 1] : 0;
-  s_sus[t] = s;
-  s_x[t] = 0ull;
-  s_rp[t] = make_int2(rp0, rp1);
-  if (t == 0) s_n = 0;
-  const bool need = s != 0ull && rp1 > rp0;
-  const int lane = t & (kWave - 1);
-  const int wave = __builtin_amdgcn_readfirstlane(t / kWave);
-  if constexpr (WT > 0) {
-    s_need[t] = need;
-    __syncthreads();
-    // one item per node: its first word in this block collects the others
-    const int q0 = wv ? static_cast<int>(w - static_cast<int64_t>(i) * a.W) : 0;
-    if (wv && (t == 0 || q0 == 0)) {
-      const int nq = min(a.W - q0, kBlock - t);
-      bool any = false;
-      for (int j = 0; j < nq; ++j) any |= s_need[t + j] != 0;
-      if (any) s_list[atomicAdd(&s_n, 1)] = t;
-    }
-    __syncthreads();
-    const int n_list = s_n;
-    for (int it = wave; it < n_list; it += kWavesPerBlock) {
-      const int tt = __builtin_amdgcn_readfirstlane(s_list[it]);
-      const int64_t ww = static_cast<int64_t>(blockIdx.x) * kBlock + tt;
-      const int ii = static_cast<int>(ww / a.W);
-      const int qq0 = static_cast<int>(ww - static_cast<int64_t>(ii) * a.W);
-      const int nq = min(a.W - qq0, kBlock - tt);
-      uint32_t acc[WT];
-      const int2 rr = s_rp[tt];
-      node_exposure<WT>(__builtin_amdgcn_readfirstlane(rr.x), __builtin_amdgcn_readfirstlane(rr.y),
-                        a.entries, a.I_cur, a.W, a.ws_rel, a.we_rel, a.rank_limit, qq0, nq, lane,
-                        acc);
+__global__ __launch_bounds__(kBlock) void expo_kernel(StepArgs a) {
+  const int lane = threadIdx.x & (kWave - 1);
+  const int i = blockIdx.x * kWavesPerBlock + __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
+  if (i >= a.N) return;
+  uint64_t *X_k = a.X + static_cast<size_t>(a.k % a.depth) * a.NW + static_cast<size_t>(i) * a.W;
+  const uint64_t *S = a.S + static_cast<size_t>(i) * a.W;
+  const int e0 = a.rowptr[i], e1 = a.rowptr[i + 1];
+  const int net = a.net_of[i];
+  const NetDev nd = a.nets[net];
+  const uint32_t rlim = a.rank_limit ? a.rank_limit[net] : UINT32_MAX;
+  for (int q0 = 0; q0 < a.W; q0 += WT) {
+    const int nq = min(WT, a.W - q0);
+    uint64_t ss[WT];
+    uint64_t any_sus = 0ull;
 #pragma unroll
-      for (int q = 0; q < WT; ++q) {
-        if (q < nq) {
-          const uint64_t ss = s_sus[tt + q];
-          if (ss != 0ull) {
-            const uint64_t x =
-                draw_word<kInjected>(a, ii, qq0 + q, lane, ss, static_cast<float>(acc[q]));
-            if (lane == 0) s_x[tt + q] = x;
-          }
-        }
-      }
+    for (int q = 0; q < WT; ++q) {
+      ss[q] = q < nq ? S[q0 + q] : 0ull;
+      any_sus |= ss[q];
     }
-    __syncthreads();
-  } else {
-  __syncthreads();
-  if (need) s_list[atomicAdd(&s_n, 1)] = t;
-  __syncthreads();
-  const int n_list = s_n;
-  for (int it = wave; it < n_list; it += kWavesPerBlock) {
-    const int tt = __builtin_amdgcn_readfirstlane(s_list[it]);
-    const int64_t ww = static_cast<int64_t>(blockIdx.x) * kBlock + tt;
-    const int ii = static_cast<int>(ww / a.W);
-    const int qq = static_cast<int>(ww - static_cast<int64_t>(ii) * a.W);
-    const uint64_t ss = s_sus[tt];
-    const float E = row_exposure<kIntMode>(a.rowptr, a.entries, a.I_cur, a.W, a.ws_rel, a.we_rel,
-                                           a.rank_limit, a.scale, ii, qq, lane);
-    const uint64_t x = draw_word<kInjected>(a, ii, qq, lane, ss, E);
-    if (lane == 0) s_x[tt] = x;
+    uint64_t x[WT];
+#pragma unroll
+    for (int q = 0; q < WT; ++q) x[q] = 0ull;
+    if (any_sus != 0ull && (e1 > e0 || kInjected)) {
+      float E[WT];
+      group_exposure<kIntMode, WT>(a, e0, e1, rlim, q0, nq, lane, E);
+#pragma unroll
+      for (int q = 0; q < WT; ++q)
+        if (q < nq && ss[q] != 0ull) x[q] = draw_word<kInjected>(a, nd, net, i, q0 + q, lane, ss[q], E[q]);
+    }
+    // lane q stores word q of the group
+    uint64_t mine = 0ull;
+#pragma unroll
+    for (int q = 0; q < WT; ++q)
+      if (lane == q) mine = x[q];
+    if (lane < nq) X_k[q0 + lane] = mine;
   }
-  __syncthreads();
-  }
-  if (wv) {
-    const uint64_t x = s_x[t];
-    a.X_k[w] = x;
-    uint64_t inf = inf_pre;
-    uint64_t sus = (s & ~x) | sus_add;
-    if (a.on_is_local) inf ^= x;
-    if (a.off_is_local) inf ^= x;
-    if (a.s_is_local) sus |= x;
+}
+
+// Phase 2 of an iteration: one thread per word applies the timer flips for
+// iteration k+1 and advances the running prefix of the infection counts,
+// P[k] = P[k-1] + inf_count[k-1].
+__global__ __launch_bounds__(kBlock) void update_kernel(StepArgs a) {
+  const int64_t w = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
+  if (w < a.NW) {
+    const int i = static_cast<int>(w / a.W);
+    const NetDev nd = a.nets[a.net_of[i]];
+    const uint64_t x = a.X[static_cast<size_t>(a.k % a.depth) * a.NW + w];
+    uint64_t inf = a.I_cur[w];
+    uint64_t sus = a.S[w] & ~x;
+    if (const uint64_t *p = ring_word(a, nd.n_on, w)) inf ^= *p;
+    if (const uint64_t *p = ring_word(a, nd.n_off, w)) inf ^= *p;
+    if (const uint64_t *p = ring_word(a, nd.n_s, w)) sus |= *p;
+    if (nd.n_on == 1) inf ^= x;
+    if (nd.n_off == 1) inf ^= x;
+    if (nd.n_s == 1) sus |= x;
+    const int32_t k1 = a.k + 1;
+    const bool fl_on = nd.m_on != kNever && nd.m_on >= 1 && k1 == nd.m_on;
+    const bool fl_off = nd.m_off != kNever && nd.m_off >= 1 && k1 == nd.m_off;
+    const bool fl_s = nd.m_s != kNever && nd.m_s >= 1 && k1 == nd.m_s;
+    if (fl_on || fl_off || fl_s) {
+      const uint64_t xi = a.X_init[w];
+      if (fl_on) inf ^= xi;
+      if (fl_off) inf ^= xi;
+      if (fl_s) sus |= xi;
+    }
     a.I_next[w] = inf;
     a.S[w] = sus;
   }
-  // running prefix of the infection counts: P[k] = P[k-1] + inf_count[k-1]
-  const int64_t rr = w;
-  if (a.k >= 1 && rr < a.M)
-    a.P[static_cast<size_t>(a.k) * a.M + rr] =
-        a.P[static_cast<size_t>(a.k - 1) * a.M + rr] +
-        a.inf_count[static_cast<size_t>(a.k - 1) * a.M + rr];
+  const int64_t G = static_cast<int64_t>(a.n_net) * a.M;
+  if (a.k >= 1 && w < G)
+    a.P[static_cast<size_t>(a.k) * G + w] =
+        a.P[static_cast<size_t>(a.k - 1) * G + w] + a.inf_count[static_cast<size_t>(a.k - 1) * G + w];
 }
 
 struct InitArgs {
@@ -307,13 +388,12 @@ struct InitArgs {
   uint64_t *I;
   uint64_t *X_init;
   int32_t *entry;
-  int32_t *init_count;  // [M]
+  int32_t *init_count;    // [n_net][M]
   const float *uniforms;  // slot 0 or null
+  const NetDev *nets;
+  const int32_t *net_of;
   uint64_t seed;
   int32_t N, M, W, Mp;
-  float p0;
-  int32_t init_susceptible;  // the seed countdown is 0 (susceptible at once)
-  int32_t init_infectious;   // the seed countdown is infectious at iteration 0
 };
 
 // cu:73-90 with a counter-based draw (slot 0): seeds, initial bit words,
@@ -326,53 +406,58 @@ __global__ __launch_bounds__(kBlock) void init_kernel(InitArgs a) {
   const bool rv = r < a.M;
   const int i = blockIdx.x * kWavesPerBlock + wave;
   if (i >= a.N) return;
+  const int net = a.net_of[i];
+  const NetDev nd = a.nets[net];
   bool seed = false;
   if (rv) {
     const float u = a.uniforms ? a.uniforms[static_cast<size_t>(r) * a.N + i]
-                               : rp::philox_uniform(a.seed, static_cast<uint32_t>(i),
-                                                    static_cast<uint32_t>(r), 0u);
-    seed = u < a.p0;
+                               : rp::philox_uniform(a.seed, static_cast<uint32_t>(i - nd.node_off),
+                                                    static_cast<uint32_t>(r), 0u, nd.stream);
+    seed = u < nd.p0;
   }
   const uint64_t xi = __ballot(seed);
   const uint64_t valid = __ballot(rv);
   a.entry[static_cast<size_t>(i) * a.Mp + r] = seed ? 0 : -1;
-  if (seed) atomicAdd(a.init_count + r, 1);
+  if (seed) atomicAdd(a.init_count + static_cast<size_t>(net) * a.M + r, 1);
   if (lane == 0) {
     const size_t w = static_cast<size_t>(i) * a.W + q;
     a.X_init[w] = xi;
-    a.S[w] = a.init_susceptible ? valid : (valid & ~xi);
-    a.I[w] = a.init_infectious ? xi : 0ull;
+    a.S[w] = nd.m_s == 0 ? valid : (valid & ~xi);
+    a.I[w] = nd.m_on == 0 ? xi : 0ull;
   }
 }
 
 // P[k] for the last completed iteration.
-__global__ void prefix_tail_kernel(int32_t *P, const int32_t *inf_count, int32_t M, int32_t k) {
-  const int r = blockIdx.x * blockDim.x + threadIdx.x;
-  if (r < M && k >= 1)
-    P[static_cast<size_t>(k) * M + r] =
-        P[static_cast<size_t>(k - 1) * M + r] + inf_count[static_cast<size_t>(k - 1) * M + r];
+__global__ void prefix_tail_kernel(int32_t *P, const int32_t *inf_count, int32_t G, int32_t k) {
+  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g < G && k >= 1)
+    P[static_cast<size_t>(k) * G + g] =
+        P[static_cast<size_t>(k - 1) * G + g] + inf_count[static_cast<size_t>(k - 1) * G + g];
 }
 
-// Counts of blocks j0..j1: exposed, infectious, resistant of the state
-// after iteration j-1 (block 0: the initial state).
+// Counts of blocks j0..j1: exposed, infectious, resistant of the state after
+// iteration j-1 (block 0: the initial state), per network and replica.
 __global__ void counts_kernel(int32_t *counts, const int32_t *P, const int32_t *init_count,
-                              const int8_t *class_init, Segs segs, int32_t M, int32_t j0,
-                              int32_t j1) {
-  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+                              const int8_t *class_init, const Segs *segs, int32_t n_net,
+                              int32_t M, int32_t T, int32_t j0, int32_t j1) {
+  const int g = blockIdx.x * blockDim.x + threadIdx.x;  // net * M + r
   const int j = j0 + blockIdx.y;
-  if (r >= M || j > j1) return;
+  const int G = n_net * M;
+  if (g >= G || j > j1) return;
+  const int net = g / M;
+  const Segs &sg = segs[net];
   int32_t c[4] = {0, 0, 0, 0};
-  for (int g = 0; g < segs.n; ++g) {
-    const int hi = j - segs.a[g] + 1;  // sum over e in [j-b+1, j-a]
-    const int lo = j - segs.b[g] + 1;
+  for (int h = 0; h < sg.n; ++h) {
+    const int hi = j - sg.a[h] + 1;  // sum over e in [j-b+1, j-a]
+    const int lo = j - sg.b[h] + 1;
     if (hi <= 0) continue;
-    const int32_t ph = P[static_cast<size_t>(hi) * M + r];
-    const int32_t pl = lo > 0 ? P[static_cast<size_t>(lo) * M + r] : 0;
-    c[segs.cls[g]] += ph - pl;
+    const int32_t ph = P[static_cast<size_t>(hi) * G + g];
+    const int32_t pl = lo > 0 ? P[static_cast<size_t>(lo) * G + g] : 0;
+    c[sg.cls[h]] += ph - pl;
   }
-  const int ci = class_init[j];
-  if (ci) c[ci] += init_count[r];
-  int32_t *dst = counts + (static_cast<size_t>(j) * M + r) * 4;
+  const int ci = class_init[static_cast<size_t>(net) * (T + 1) + j];
+  if (ci) c[ci] += init_count[g];
+  int32_t *dst = counts + (static_cast<size_t>(j) * G + g) * 4;
   dst[0] = c[1];
   dst[1] = c[2];
   dst[2] = c[3];
@@ -388,8 +473,9 @@ __global__ __launch_bounds__(kBlock) void exposure_kernel(StepArgs a, float *out
   const int r = q * kWave + lane;
   const int i = blockIdx.x * kWavesPerBlock + wave;
   if (i >= a.N) return;
-  const float E = row_exposure<kIntMode>(a.rowptr, a.entries, a.I_cur, a.W, a.ws_rel, a.we_rel,
-                                         a.rank_limit, a.scale, i, q, lane);
+  const uint32_t rlim = a.rank_limit ? a.rank_limit[a.net_of[i]] : UINT32_MAX;
+  const float E = row_exposure<kIntMode>(a.rowptr[i], a.rowptr[i + 1], a.entries, a.I_cur, a.W,
+                                         a.ws_rel, a.we_rel, rlim, a.scale, q, lane);
   if (r < a.M) out[static_cast<size_t>(r) * a.N + i] = E;
 }
 
@@ -398,6 +484,7 @@ __global__ void slice_kernel(StepArgs a, int pass, const int32_t *out_rowptr, in
                              int32_t *colind, float *val) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= a.N) return;
+  const uint32_t rlim = a.rank_limit ? a.rank_limit[a.net_of[i]] : UINT32_MAX;
   const int e0 = a.rowptr[i], e1 = a.rowptr[i + 1];
   int n = 0;
   int e = e0;
@@ -408,7 +495,7 @@ __global__ void slice_kernel(StepArgs a, int pass, const int32_t *out_rowptr, in
     for (; e < e1 && a.entries[e].x == col; ++e) {
       const int4 en = a.entries[e];
       const int ov = min(en.z, a.we_rel) - max(en.y, a.ws_rel);
-      if (ov > 0 && static_cast<uint32_t>(en.w) <= a.rank_limit) {
+      if (ov > 0 && static_cast<uint32_t>(en.w) <= rlim) {
         const float wc = static_cast<float>(ov) * a.scale;
         w = any ? w + wc : wc;
         any = true;
@@ -439,43 +526,65 @@ int host_class(int32_t v, int32_t upper, int32_t Ed, int32_t Id, int32_t Rd) {
 
 }  // namespace
 
-// Timer tables of one context (see the header comment).
-struct TimerPlan {
+// One network of a context, on the host.
+struct NetHost {
+  rp::Network net;
+  rp::Plan plan;
+  rp::Slices slices;
+  float p0 = 0.0f, beta = 0.0f, log_base = 0.0f;
+  int32_t Ed = 0, Id = 0, Rd = 0, upper = 0, medium = 0, lower = 0;
+  uint32_t stream = 0;
+  int32_t node_off = 0;
+  // timer tables
   std::vector<int32_t> lut_new, lut_init;  // c(n), n = 0..T
   int32_t n_on = kNever, n_off = kNever, n_s = kNever;
   int32_t m_on = kNever, m_off = kNever, m_s = kNever;
-  int32_t depth = 1;
   Segs segs{};
   std::vector<int8_t> class_init;  // class of c_init(j), j = 0..T
 };
 
 struct replay_ctx {
   replay_params p{};
-  rp::Network net;
-  rp::Plan plan;
-  rp::Slices slices;
-  TimerPlan tp;
+  std::vector<NetHost> nets;
+  int32_t n_net = 0;
+  int32_t N = 0, M = 0, W = 0, Mp = 0, T = 0, depth = 1;
+  // merged plan and slices
+  std::vector<int64_t> iter_time;
+  std::vector<int32_t> iter_window;
+  std::vector<int64_t> win_ws, win_we;
+  std::vector<float> win_scale;
+  std::vector<int32_t> win_epoch;
+  std::vector<uint8_t> win_int;
+  std::vector<int64_t> epoch_t0, epoch_entry0;
+  int64_t n_entries = 0;
+  int32_t windows_per_epoch = 1;
+  bool any_cut = false;
+  int truncated = 0;
+  int64_t n_contacts = 0, nlists = 0;
+  int32_t variant = REPLAY_V_WORD;
   hipStream_t stream = nullptr;
+  hipGraphExec_t graph = nullptr;
+  // device
   int4 *d_entries = nullptr;
   int32_t *d_rowptr = nullptr;
   uint64_t *d_S = nullptr;
   uint64_t *d_I[2] = {nullptr, nullptr};
-  uint64_t *d_X = nullptr;       // ring, depth x N x W
+  uint64_t *d_X = nullptr;
   uint64_t *d_Xinit = nullptr;
   int32_t *d_entry = nullptr;
-  int32_t *d_inf = nullptr;      // [T+1][M]
-  int32_t *d_P = nullptr;        // [T+2][M]
+  int32_t *d_inf = nullptr;
+  int32_t *d_P = nullptr;
   int32_t *d_init_count = nullptr;
-  int32_t *d_counts = nullptr;   // [T+1][M][4]
+  int32_t *d_counts = nullptr;
   int8_t *d_class_init = nullptr;
+  Segs *d_segs = nullptr;
+  NetDev *d_nets = nullptr;
+  int32_t *d_net_of = nullptr;
+  uint32_t *d_rank_limit = nullptr;  // [n_windows][n_net] when a cap bites
   float *d_uniforms = nullptr;
-  int32_t N = 0, M = 0, W = 0, Mp = 0;
   int cur = 0;
   int32_t steps_done = 0;
-  int32_t counts_done = -1;      // last counts block computed
-  float log_base = 0.0f;
-  int32_t upper = 0, medium = 0, lower = 0;
-  int truncated = 0;
+  int32_t counts_done = -1;
   std::string err;
 };
 
@@ -505,38 +614,47 @@ int dev_alloc(replay_ctx *ctx, T **ptr, size_t count) {
   return REPLAY_OK;
 }
 
+template <typename T>
+int upload(replay_ctx *ctx, T **ptr, const T *src, size_t count) {
+  int rc = dev_alloc(ctx, ptr, count);
+  if (rc) return rc;
+  if (count)
+    HIP_OR_FAIL(ctx, hipMemcpyAsync(*ptr, src, count * sizeof(T), hipMemcpyHostToDevice,
+                                    ctx->stream));
+  return REPLAY_OK;
+}
+
 void release(replay_ctx *ctx) {
   if (!ctx) return;
   if (ctx->stream) hipStreamSynchronize(ctx->stream);
-  void *bufs[] = {ctx->d_entries, ctx->d_rowptr, ctx->d_S, ctx->d_I[0], ctx->d_I[1],
-                  ctx->d_X, ctx->d_Xinit, ctx->d_entry, ctx->d_inf, ctx->d_P,
-                  ctx->d_init_count, ctx->d_counts, ctx->d_class_init, ctx->d_uniforms};
+  if (ctx->graph) hipGraphExecDestroy(ctx->graph);
+  ctx->graph = nullptr;
+  void *bufs[] = {ctx->d_entries, ctx->d_rowptr, ctx->d_S,      ctx->d_I[0],        ctx->d_I[1],
+                  ctx->d_X,       ctx->d_Xinit,  ctx->d_entry,  ctx->d_inf,         ctx->d_P,
+                  ctx->d_init_count, ctx->d_counts, ctx->d_class_init, ctx->d_segs, ctx->d_nets,
+                  ctx->d_net_of,  ctx->d_rank_limit, ctx->d_uniforms};
   for (void *b : bufs) hipFree(b);
   if (ctx->stream) hipStreamDestroy(ctx->stream);
   ctx->stream = nullptr;
 }
 
-// Tabulates the countdown sequences and the offsets derived from them.
-int plan_timers(replay_ctx *ctx) {
-  TimerPlan &tp = ctx->tp;
-  const int32_t T = ctx->p.iterations;
-  const float step = ctx->p.step_size;
+// Tabulates one network's countdown sequences and the offsets derived from
+// them (see the header comment).
+int plan_timers(int32_t T, float step, NetHost &nh, std::string &err) {
   auto f = [step](int32_t c) {  // cu:169-174
     if (c > 0) c = static_cast<int32_t>(static_cast<float>(c) - step);
     if (c < 0) c = 0;
     return c;
   };
-  const int32_t c_new = static_cast<int32_t>(static_cast<float>(ctx->upper) + step);
-  const int32_t c_init = static_cast<int32_t>(static_cast<float>(ctx->medium) + step);
-  tp.lut_new.assign(static_cast<size_t>(T) + 1, 0);
-  tp.lut_init.assign(static_cast<size_t>(T) + 1, 0);
-  tp.lut_new[0] = c_new;
-  tp.lut_init[0] = c_init;
+  nh.lut_new.assign(static_cast<size_t>(T) + 1, 0);
+  nh.lut_init.assign(static_cast<size_t>(T) + 1, 0);
+  nh.lut_new[0] = static_cast<int32_t>(static_cast<float>(nh.upper) + step);   // cu:155
+  nh.lut_init[0] = static_cast<int32_t>(static_cast<float>(nh.medium) + step); // cu:85
   for (int32_t n = 1; n <= T; ++n) {
-    tp.lut_new[n] = f(tp.lut_new[n - 1]);
-    tp.lut_init[n] = f(tp.lut_init[n - 1]);
+    nh.lut_new[n] = f(nh.lut_new[n - 1]);
+    nh.lut_init[n] = f(nh.lut_init[n - 1]);
   }
-  auto infectious = [ctx](int32_t c) { return c > ctx->lower && c <= ctx->medium; };
+  auto infectious = [&nh](int32_t c) { return c > nh.lower && c <= nh.medium; };
   // contiguous infectious range [on, off) and first zero, over n >= n0
   auto ranges = [&](const std::vector<int32_t> &lut, int32_t n0, int32_t *on, int32_t *off,
                     int32_t *zero) {
@@ -550,61 +668,48 @@ int plan_timers(replay_ctx *ctx) {
     }
     return true;
   };
-  if (!ranges(tp.lut_new, 1, &tp.n_on, &tp.n_off, &tp.n_s) ||
-      !ranges(tp.lut_init, 0, &tp.m_on, &tp.m_off, &tp.m_s))
-    return fail(ctx, REPLAY_E_UNSUPPORTED, "infectious period is not contiguous");
-  int32_t dmax = 1;
-  for (int32_t n : {tp.n_on, tp.n_off, tp.n_s})
-    if (n != kNever) dmax = std::max(dmax, n);
-  tp.depth = dmax;  // slots k+1-n for 2 <= n <= dmax and slot k: dmax slots
-  // class runs of new infections over n = 1..T, and the seeds' class table
-  tp.segs.n = 0;
+  if (!ranges(nh.lut_new, 1, &nh.n_on, &nh.n_off, &nh.n_s) ||
+      !ranges(nh.lut_init, 0, &nh.m_on, &nh.m_off, &nh.m_s)) {
+    err = "infectious period is not contiguous";
+    return REPLAY_E_UNSUPPORTED;
+  }
+  nh.segs.n = 0;
   int32_t prev = -1;
   for (int32_t n = 1; n <= T; ++n) {
-    const int cls = host_class(tp.lut_new[n], ctx->upper, ctx->p.exposed_duration,
-                               ctx->p.infectious_duration, ctx->p.resistant_duration);
+    const int cls = host_class(nh.lut_new[n], nh.upper, nh.Ed, nh.Id, nh.Rd);
     if (cls != prev) {
-      if (prev > 0) tp.segs.b[tp.segs.n - 1] = n;
+      if (prev > 0) nh.segs.b[nh.segs.n - 1] = n;
       if (cls > 0) {
-        if (tp.segs.n >= kMaxSegs) return fail(ctx, REPLAY_E_UNSUPPORTED, "class schedule too long");
-        tp.segs.a[tp.segs.n] = n;
-        tp.segs.cls[tp.segs.n] = cls;
-        tp.segs.b[tp.segs.n] = T + 1;
-        tp.segs.n++;
+        if (nh.segs.n >= kMaxSegs) {
+          err = "class schedule too long";
+          return REPLAY_E_UNSUPPORTED;
+        }
+        nh.segs.a[nh.segs.n] = n;
+        nh.segs.cls[nh.segs.n] = cls;
+        nh.segs.b[nh.segs.n] = T + 1;
+        nh.segs.n++;
       }
       prev = cls;
     }
   }
-  tp.class_init.assign(static_cast<size_t>(T) + 1, 0);
+  nh.class_init.assign(static_cast<size_t>(T) + 1, 0);
   for (int32_t j = 0; j <= T; ++j)
-    tp.class_init[j] = static_cast<int8_t>(host_class(tp.lut_init[j], ctx->upper,
-                                                      ctx->p.exposed_duration,
-                                                      ctx->p.infectious_duration,
-                                                      ctx->p.resistant_duration));
+    nh.class_init[j] = static_cast<int8_t>(host_class(nh.lut_init[j], nh.upper, nh.Ed, nh.Id, nh.Rd));
   return REPLAY_OK;
 }
 
 StepArgs step_args(replay_ctx *ctx, int32_t k) {
-  const rp::Window &w = ctx->plan.windows[ctx->plan.iter_window[k]];
-  const int e = w.epoch;
-  const int64_t t0 = ctx->slices.epoch_t0[e];
-  const TimerPlan &tp = ctx->tp;
-  const size_t NW = static_cast<size_t>(ctx->N) * ctx->W;
-  auto ring = [&](int32_t n) -> const uint64_t * {  // X(k+1-n) for n >= 2
-    if (n == kNever || n < 2 || k + 1 - n < 0) return nullptr;
-    return ctx->d_X + static_cast<size_t>((k + 1 - n) % tp.depth) * NW;
-  };
+  const int32_t win = ctx->iter_window[k];
+  const int e = ctx->win_epoch[win];
+  const int64_t t0 = ctx->epoch_t0[e];
   StepArgs a;
   std::memset(&a, 0, sizeof a);
   a.rowptr = ctx->d_rowptr + static_cast<size_t>(e) * (ctx->N + 1);
-  a.entries = ctx->d_entries + ctx->slices.epoch_entry0[e];
+  a.entries = ctx->d_entries + ctx->epoch_entry0[e];
   a.I_cur = ctx->d_I[ctx->cur];
   a.I_next = ctx->d_I[ctx->cur ^ 1];
   a.S = ctx->d_S;
-  a.X_k = ctx->d_X + static_cast<size_t>(k % tp.depth) * NW;
-  a.X_on = ring(tp.n_on);
-  a.X_off = ring(tp.n_off);
-  a.X_s = ring(tp.n_s);
+  a.X = ctx->d_X;
   a.X_init = ctx->d_Xinit;
   a.entry = ctx->d_entry;
   a.inf_count = ctx->d_inf;
@@ -612,55 +717,82 @@ StepArgs step_args(replay_ctx *ctx, int32_t k) {
   a.uniforms = ctx->d_uniforms
                    ? ctx->d_uniforms + static_cast<size_t>(k + 1) * ctx->M * ctx->N
                    : nullptr;
+  a.nets = ctx->d_nets;
+  a.net_of = ctx->d_net_of;
+  a.rank_limit = ctx->any_cut ? ctx->d_rank_limit + static_cast<size_t>(win) * ctx->n_net : nullptr;
   a.seed = ctx->p.seed;
-  a.NW = static_cast<int64_t>(NW);
+  a.NW = static_cast<int64_t>(ctx->N) * ctx->W;
   a.N = ctx->N;
   a.M = ctx->M;
   a.W = ctx->W;
   a.Mp = ctx->Mp;
+  a.n_net = ctx->n_net;
+  a.depth = ctx->depth;
   a.k = k;
-  a.ws_rel = static_cast<int32_t>(w.ws - t0);
-  a.we_rel = static_cast<int32_t>(w.we - t0);
-  a.rank_limit = w.rank_limit;
-  a.scale = w.scale;
-  a.log_base = ctx->log_base;
+  a.ws_rel = static_cast<int32_t>(ctx->win_ws[win] - t0);
+  a.we_rel = static_cast<int32_t>(ctx->win_we[win] - t0);
+  a.scale = ctx->win_scale[win];
   a.slot = static_cast<uint32_t>(k + 1);
-  a.on_is_local = tp.n_on == 1;
-  a.off_is_local = tp.n_off == 1;
-  a.s_is_local = tp.n_s == 1;
-  a.init_flip_I = (tp.m_on != kNever && tp.m_on >= 1 && k + 1 == tp.m_on ? 1 : 0) |
-                  (tp.m_off != kNever && tp.m_off >= 1 && k + 1 == tp.m_off ? 2 : 0);
-  a.init_set_S = (tp.m_s != kNever && tp.m_s >= 1 && k + 1 == tp.m_s) ? 1 : 0;
   return a;
 }
 
 template <bool kInt, bool kInj>
-void launch_step_w(const StepArgs &a, dim3 grid, hipStream_t st) {
-  if (kInt && a.W <= 4)
-    hipLaunchKernelGGL((step_kernel<kInt, kInj, 4>), grid, dim3(kBlock), 0, st, a);
-  else if (kInt && a.W <= 16)
-    hipLaunchKernelGGL((step_kernel<kInt, kInj, 16>), grid, dim3(kBlock), 0, st, a);
+void launch_expo(const StepArgs &a, int variant, hipStream_t st) {
+  const dim3 grid((a.N + kWavesPerBlock - 1) / kWavesPerBlock);
+  if (variant == REPLAY_V_NODE16)
+    hipLaunchKernelGGL((expo_kernel<kInt, kInj, 16>), grid, dim3(kBlock), 0, st, a);
+  else if (variant == REPLAY_V_NODE4)
+    hipLaunchKernelGGL((expo_kernel<kInt, kInj, 4>), grid, dim3(kBlock), 0, st, a);
   else
-    hipLaunchKernelGGL((step_kernel<kInt, kInj, 0>), grid, dim3(kBlock), 0, st, a);
+    hipLaunchKernelGGL((expo_kernel<kInt, kInj, 1>), grid, dim3(kBlock), 0, st, a);
 }
 
-void launch_step(const StepArgs &a, bool int_mode, bool injected, dim3 grid, hipStream_t st) {
+// One iteration: exposure + draws per node, then the word flips.
+void launch_step(const StepArgs &a, int variant, bool int_mode, bool injected, dim3 grid,
+                 hipStream_t st) {
   if (int_mode && !injected)
-    launch_step_w<true, false>(a, grid, st);
+    launch_expo<true, false>(a, variant, st);
   else if (int_mode)
-    launch_step_w<true, true>(a, grid, st);
+    launch_expo<true, true>(a, variant, st);
   else if (!injected)
-    launch_step_w<false, false>(a, grid, st);
+    launch_expo<false, false>(a, variant, st);
   else
-    launch_step_w<false, true>(a, grid, st);
+    launch_expo<false, true>(a, variant, st);
+  hipLaunchKernelGGL(update_kernel, grid, dim3(kBlock), 0, st, a);
+}
+
+dim3 step_grid(const replay_ctx *ctx) {
+  // threads cover every word and, for the running prefix, every replica
+  const int64_t NW = static_cast<int64_t>(ctx->N) * ctx->W;
+  const int64_t cover = std::max<int64_t>(NW, static_cast<int64_t>(ctx->n_net) * ctx->M);
+  return dim3(static_cast<unsigned>((cover + kBlock - 1) / kBlock));
+}
+
+bool variant_applies(const replay_ctx *ctx, int v) {
+  (void)ctx;
+  return v == REPLAY_V_WORD || v == REPLAY_V_NODE4 || v == REPLAY_V_NODE16;
+}
+
+int enqueue_steps(replay_ctx *ctx, int32_t steps) {
+  const dim3 grid = step_grid(ctx);
+  for (int32_t s = 0; s < steps; ++s) {
+    const int32_t k = ctx->steps_done;
+    StepArgs a = step_args(ctx, k);
+    const bool int_mode = ctx->win_int[ctx->iter_window[k]] != 0;
+    launch_step(a, ctx->variant, int_mode, a.uniforms != nullptr, grid, ctx->stream);
+    HIP_OR_FAIL(ctx, hipGetLastError());
+    ctx->cur ^= 1;
+    ctx->steps_done = k + 1;
+  }
+  return REPLAY_OK;
 }
 
 int do_reset(replay_ctx *ctx) {
-  const int32_t T = ctx->p.iterations;
-  const size_t M = static_cast<size_t>(ctx->M);
-  HIP_OR_FAIL(ctx, hipMemsetAsync(ctx->d_inf, 0, (T + 1) * M * sizeof(int32_t), ctx->stream));
-  HIP_OR_FAIL(ctx, hipMemsetAsync(ctx->d_P, 0, (T + 2) * M * sizeof(int32_t), ctx->stream));
-  HIP_OR_FAIL(ctx, hipMemsetAsync(ctx->d_init_count, 0, M * sizeof(int32_t), ctx->stream));
+  const size_t G = static_cast<size_t>(ctx->n_net) * ctx->M;
+  const size_t T = static_cast<size_t>(ctx->T);
+  HIP_OR_FAIL(ctx, hipMemsetAsync(ctx->d_inf, 0, (T + 1) * G * sizeof(int32_t), ctx->stream));
+  HIP_OR_FAIL(ctx, hipMemsetAsync(ctx->d_P, 0, (T + 2) * G * sizeof(int32_t), ctx->stream));
+  HIP_OR_FAIL(ctx, hipMemsetAsync(ctx->d_init_count, 0, G * sizeof(int32_t), ctx->stream));
   InitArgs a;
   std::memset(&a, 0, sizeof a);
   a.S = ctx->d_S;
@@ -669,14 +801,13 @@ int do_reset(replay_ctx *ctx) {
   a.entry = ctx->d_entry;
   a.init_count = ctx->d_init_count;
   a.uniforms = ctx->d_uniforms;
+  a.nets = ctx->d_nets;
+  a.net_of = ctx->d_net_of;
   a.seed = ctx->p.seed;
   a.N = ctx->N;
   a.M = ctx->M;
   a.W = ctx->W;
   a.Mp = ctx->Mp;
-  a.p0 = ctx->p.initial_infected;
-  a.init_susceptible = ctx->tp.m_s == 0;
-  a.init_infectious = ctx->tp.m_on == 0;
   dim3 grid((ctx->N + kWavesPerBlock - 1) / kWavesPerBlock, ctx->W);
   hipLaunchKernelGGL(init_kernel, grid, dim3(kBlock), 0, ctx->stream, a);
   HIP_OR_FAIL(ctx, hipGetLastError());
@@ -690,67 +821,15 @@ int do_reset(replay_ctx *ctx) {
 int enqueue_counts(replay_ctx *ctx) {
   const int32_t j0 = ctx->counts_done + 1, j1 = ctx->steps_done;
   if (j0 > j1) return REPLAY_OK;
-  const int tb = 256, nb = (ctx->M + tb - 1) / tb;
+  const int G = ctx->n_net * ctx->M;
+  const int tb = 256, nb = (G + tb - 1) / tb;
   hipLaunchKernelGGL(prefix_tail_kernel, dim3(nb), dim3(tb), 0, ctx->stream, ctx->d_P, ctx->d_inf,
-                     ctx->M, j1);
+                     G, j1);
   hipLaunchKernelGGL(counts_kernel, dim3(nb, j1 - j0 + 1), dim3(tb), 0, ctx->stream,
-                     ctx->d_counts, ctx->d_P, ctx->d_init_count, ctx->d_class_init, ctx->tp.segs,
-                     ctx->M, j0, j1);
+                     ctx->d_counts, ctx->d_P, ctx->d_init_count, ctx->d_class_init, ctx->d_segs,
+                     ctx->n_net, ctx->M, ctx->T, j0, j1);
   HIP_OR_FAIL(ctx, hipGetLastError());
   ctx->counts_done = j1;
-  return REPLAY_OK;
-}
-
-int finish_open(replay_ctx *ctx, const replay_params *p) {
-  std::string err;
-  int rc = rp::finalize(ctx->net, err);
-  if (rc != REPLAY_OK) return fail(ctx, rc, err);
-  rc = rp::plan_windows(ctx->net, *p, ctx->plan, err);
-  if (rc != REPLAY_OK) return fail(ctx, rc, err);
-  int64_t budget = p->max_slice_bytes > 0 ? p->max_slice_bytes : (int64_t(32) << 30);
-  rc = rp::build_slices(ctx->net, ctx->plan, p->windows_per_epoch, budget, ctx->slices, err);
-  if (rc != REPLAY_OK) return fail(ctx, rc, err);
-  for (const auto &w : ctx->plan.windows) ctx->truncated += (w.rank_limit != UINT32_MAX);
-  ctx->N = ctx->net.n_nodes;
-  ctx->M = p->M;
-  ctx->W = (p->M + kWave - 1) / kWave;
-  ctx->Mp = ctx->W * kWave;
-  ctx->upper = p->exposed_duration + p->infectious_duration + p->resistant_duration;
-  ctx->medium = p->infectious_duration + p->resistant_duration;
-  ctx->lower = p->resistant_duration;
-  ctx->log_base = logf(1.0f - p->infect_prob);
-  rc = plan_timers(ctx);
-  if (rc != REPLAY_OK) return rc;
-  HIP_OR_FAIL(ctx, hipSetDevice(p->device));
-  HIP_OR_FAIL(ctx, hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
-  const size_t nE = ctx->slices.entries.size();
-  const size_t nR = ctx->slices.rowptr.size();
-  const size_t NW = static_cast<size_t>(ctx->N) * ctx->W;
-  const size_t T = static_cast<size_t>(p->iterations);
-  const size_t M = static_cast<size_t>(ctx->M);
-  if ((rc = dev_alloc(ctx, &ctx->d_entries, nE)) || (rc = dev_alloc(ctx, &ctx->d_rowptr, nR)) ||
-      (rc = dev_alloc(ctx, &ctx->d_S, NW)) || (rc = dev_alloc(ctx, &ctx->d_I[0], NW)) ||
-      (rc = dev_alloc(ctx, &ctx->d_I[1], NW)) ||
-      (rc = dev_alloc(ctx, &ctx->d_X, NW * static_cast<size_t>(ctx->tp.depth))) ||
-      (rc = dev_alloc(ctx, &ctx->d_Xinit, NW)) ||
-      (rc = dev_alloc(ctx, &ctx->d_entry, static_cast<size_t>(ctx->N) * ctx->Mp)) ||
-      (rc = dev_alloc(ctx, &ctx->d_inf, (T + 1) * M)) ||
-      (rc = dev_alloc(ctx, &ctx->d_P, (T + 2) * M)) ||
-      (rc = dev_alloc(ctx, &ctx->d_init_count, M)) ||
-      (rc = dev_alloc(ctx, &ctx->d_counts, (T + 1) * M * 4)) ||
-      (rc = dev_alloc(ctx, &ctx->d_class_init, T + 1)))
-    return rc;
-  if (nE)
-    HIP_OR_FAIL(ctx, hipMemcpyAsync(ctx->d_entries, ctx->slices.entries.data(), nE * sizeof(int4),
-                                    hipMemcpyHostToDevice, ctx->stream));
-  if (nR)
-    HIP_OR_FAIL(ctx, hipMemcpyAsync(ctx->d_rowptr, ctx->slices.rowptr.data(),
-                                    nR * sizeof(int32_t), hipMemcpyHostToDevice, ctx->stream));
-  HIP_OR_FAIL(ctx, hipMemcpyAsync(ctx->d_class_init, ctx->tp.class_init.data(), T + 1,
-                                  hipMemcpyHostToDevice, ctx->stream));
-  rc = do_reset(ctx);
-  if (rc != REPLAY_OK) return rc;
-  HIP_OR_FAIL(ctx, hipStreamSynchronize(ctx->stream));
   return REPLAY_OK;
 }
 
@@ -761,16 +840,219 @@ int check_params(const replay_params *p) {
   if (p->static_network_duration <= 0)
     return fail(nullptr, REPLAY_E_ARG, "static network duration must be positive");
   if (!(p->step_size > 0.0f)) return fail(nullptr, REPLAY_E_ARG, "step size must be positive");
-  if (p->exposed_duration < 0 || p->infectious_duration < 0 || p->resistant_duration < 0)
-    return fail(nullptr, REPLAY_E_ARG, "durations must be >= 0");
-  const int64_t up = static_cast<int64_t>(p->exposed_duration) + p->infectious_duration +
-                     p->resistant_duration;
-  if (up > INT32_MAX) return fail(nullptr, REPLAY_E_ARG, "durations overflow int32");
   return REPLAY_OK;
 }
 
-int open_common(replay_ctx *ctx, const replay_params *p, replay_ctx **out) {
-  int rc = finish_open(ctx, p);
+int check_durations(int32_t Ed, int32_t Id, int32_t Rd) {
+  if (Ed < 0 || Id < 0 || Rd < 0) return fail(nullptr, REPLAY_E_ARG, "durations must be >= 0");
+  if (static_cast<int64_t>(Ed) + Id + Rd > INT32_MAX)
+    return fail(nullptr, REPLAY_E_ARG, "durations overflow int32");
+  return REPLAY_OK;
+}
+
+// Runs fn(0..n-1) on up to 16 host threads.
+template <typename F>
+void parallel_for(size_t n, F fn) {
+  const size_t nt = std::min<size_t>(
+      n, std::max<size_t>(1, std::min<unsigned>(16, std::thread::hardware_concurrency())));
+  if (nt <= 1) {
+    for (size_t i = 0; i < n; ++i) fn(i);
+    return;
+  }
+  std::atomic<size_t> next{0};
+  std::vector<std::thread> th;
+  for (size_t t = 0; t < nt; ++t)
+    th.emplace_back([&] {
+      for (size_t i; (i = next.fetch_add(1)) < n;) fn(i);
+    });
+  for (auto &x : th) x.join();
+}
+
+// Picks the epoch size for all networks together.
+int32_t pick_epoch(replay_ctx *ctx) {
+  if (ctx->p.windows_per_epoch > 0) return ctx->p.windows_per_epoch;
+  const int64_t budget = ctx->p.max_slice_bytes > 0 ? ctx->p.max_slice_bytes : (int64_t(8) << 30);
+  const int32_t W = static_cast<int32_t>(ctx->win_ws.size());
+  std::vector<int64_t> cand;
+  for (int64_t S = 1; S < W; S *= 2) cand.push_back(S);
+  cand.push_back(std::max<int32_t>(W, 1));
+  std::vector<std::vector<int64_t>> per(ctx->nets.size());
+  parallel_for(ctx->nets.size(),
+               [&](size_t n) { per[n] = rp::slice_bytes(ctx->nets[n].net, ctx->nets[n].plan, cand); });
+  std::vector<int64_t> total(cand.size(), 0);
+  for (const auto &b : per)
+    for (size_t j = 0; j < cand.size(); ++j) total[j] += b[j];
+  for (size_t j = 0; j < cand.size(); ++j)
+    if (total[j] <= budget) return static_cast<int32_t>(cand[j]);
+  return static_cast<int32_t>(cand.back());
+}
+
+int finish_open(replay_ctx *ctx) {
+  std::string err;
+  const replay_params &p = ctx->p;
+  ctx->T = p.iterations;
+  ctx->M = p.M;
+  ctx->W = (p.M + kWave - 1) / kWave;
+  ctx->Mp = ctx->W * kWave;
+  ctx->n_net = static_cast<int32_t>(ctx->nets.size());
+  // per network, independent, on host threads: rank order, cap, windows,
+  // timers
+  const size_t n_nets = ctx->nets.size();
+  std::vector<int> rcs(n_nets, REPLAY_OK);
+  std::vector<std::string> errs(n_nets);
+  parallel_for(n_nets, [&](size_t n) {
+    NetHost &nh = ctx->nets[n];
+    int rc = rp::finalize(nh.net, errs[n]);
+    if (rc == REPLAY_OK) rc = rp::plan_windows(nh.net, p, nh.plan, errs[n]);
+    if (rc == REPLAY_OK) {
+      nh.upper = nh.Ed + nh.Id + nh.Rd;
+      nh.medium = nh.Id + nh.Rd;
+      nh.lower = nh.Rd;
+      nh.log_base = logf(1.0f - nh.beta);
+      rc = plan_timers(ctx->T, p.step_size, nh, errs[n]);
+    }
+    rcs[n] = rc;
+  });
+  for (size_t n = 0; n < n_nets; ++n)
+    if (rcs[n] != REPLAY_OK)
+      return fail(ctx, rcs[n], (n_nets > 1 ? "network " + std::to_string(n) + ": " : "") + errs[n]);
+  int32_t off = 0;
+  for (auto &nh : ctx->nets) {
+    nh.node_off = off;
+    off += nh.net.n_nodes;
+    for (int32_t n : {nh.n_on, nh.n_off, nh.n_s})
+      if (n != kNever) ctx->depth = std::max(ctx->depth, n);
+    ctx->n_contacts += static_cast<int64_t>(nh.net.contacts.size());
+    ctx->nlists = std::max(ctx->nlists, nh.net.nlists);
+  }
+  ctx->N = off;
+  if (static_cast<int64_t>(ctx->N) * ctx->W > INT32_MAX)
+    return fail(ctx, REPLAY_E_UNSUPPORTED, "more than 2^31 replica words");
+  // windows are shared: every network planned the same iterations
+  const rp::Plan &p0 = ctx->nets[0].plan;
+  ctx->iter_time = p0.iter_time;
+  ctx->iter_window = p0.iter_window;
+  const int32_t nW = static_cast<int32_t>(p0.windows.size());
+  for (const auto &w : p0.windows) {
+    ctx->win_ws.push_back(w.ws);
+    ctx->win_we.push_back(w.we);
+    ctx->win_scale.push_back(w.scale);
+  }
+  // slices with one epoch size for all networks, merged row-wise
+  const int32_t S = pick_epoch(ctx);
+  parallel_for(n_nets, [&](size_t n) {
+    NetHost &nh = ctx->nets[n];
+    rcs[n] = rp::build_slices(nh.net, nh.plan, S, INT64_MAX, nh.slices, errs[n]);
+  });
+  for (size_t n = 0; n < n_nets; ++n)
+    if (rcs[n] != REPLAY_OK) return fail(ctx, rcs[n], errs[n]);
+  ctx->windows_per_epoch = nW ? ctx->nets[0].slices.windows_per_epoch : 1;
+  ctx->epoch_t0 = ctx->nets[0].slices.epoch_t0;
+  const int32_t n_ep = static_cast<int32_t>(ctx->epoch_t0.size());
+  ctx->win_epoch.assign(static_cast<size_t>(nW), 0);
+  ctx->win_int.assign(static_cast<size_t>(nW), 1);
+  std::vector<uint32_t> limits(static_cast<size_t>(nW) * ctx->n_net, UINT32_MAX);
+  for (int32_t w = 0; w < nW; ++w) {
+    ctx->win_epoch[w] = ctx->nets[0].plan.windows[w].epoch;
+    bool cut = false;
+    for (int32_t n = 0; n < ctx->n_net; ++n) {
+      const rp::Window &nw = ctx->nets[n].plan.windows[w];
+      ctx->win_int[w] &= static_cast<uint8_t>(nw.int_mode != 0);
+      limits[static_cast<size_t>(w) * ctx->n_net + n] = nw.rank_limit;
+      cut |= nw.rank_limit != UINT32_MAX;
+    }
+    ctx->truncated += cut;
+  }
+  ctx->any_cut = ctx->truncated > 0;
+  std::vector<int32_t> rowptr(static_cast<size_t>(n_ep) * (ctx->N + 1));
+  ctx->epoch_entry0.assign(static_cast<size_t>(n_ep) + 1, 0);
+  int64_t total = 0;
+  for (int32_t e = 0; e < n_ep; ++e) {
+    ctx->epoch_entry0[e] = total;
+    int64_t base = 0;  // entries of earlier networks in this epoch
+    for (auto &nh : ctx->nets) {
+      const int32_t Nn = nh.net.n_nodes;
+      const int32_t *rp = &nh.slices.rowptr[static_cast<size_t>(e) * (Nn + 1)];
+      for (int32_t r = 0; r < Nn; ++r)
+        rowptr[static_cast<size_t>(e) * (ctx->N + 1) + nh.node_off + r] =
+            static_cast<int32_t>(base + rp[r]);
+      base += rp[Nn];
+      if (base > INT32_MAX) return fail(ctx, REPLAY_E_UNSUPPORTED, "epoch holds more than 2^31 entries");
+    }
+    rowptr[static_cast<size_t>(e) * (ctx->N + 1) + ctx->N] = static_cast<int32_t>(base);
+    total += base;
+  }
+  ctx->epoch_entry0[n_ep] = total;
+  ctx->n_entries = total;
+  std::vector<rp::Entry> entries(static_cast<size_t>(total));
+  for (int32_t e = 0; e < n_ep; ++e) {
+    int64_t dst = ctx->epoch_entry0[e];
+    for (auto &nh : ctx->nets) {
+      const int64_t b0 = nh.slices.epoch_entry0[e], b1 = nh.slices.epoch_entry0[e + 1];
+      for (int64_t q = b0; q < b1; ++q) {
+        rp::Entry en = nh.slices.entries[static_cast<size_t>(q)];
+        en.col += nh.node_off;
+        entries[static_cast<size_t>(dst++)] = en;
+      }
+    }
+  }
+  for (auto &nh : ctx->nets) {  // per-network copies are no longer needed
+    std::vector<rp::Entry>().swap(nh.slices.entries);
+    std::vector<int32_t>().swap(nh.slices.rowptr);
+  }
+  // device tables
+  std::vector<NetDev> nd(static_cast<size_t>(ctx->n_net));
+  std::vector<Segs> segs(static_cast<size_t>(ctx->n_net));
+  std::vector<int8_t> class_init(static_cast<size_t>(ctx->n_net) * (ctx->T + 1));
+  std::vector<int32_t> net_of(static_cast<size_t>(ctx->N));
+  for (int32_t n = 0; n < ctx->n_net; ++n) {
+    const NetHost &nh = ctx->nets[n];
+    nd[n] = NetDev{nh.log_base, nh.p0, nh.node_off, nh.stream, nh.n_on, nh.n_off, nh.n_s,
+                   nh.m_on, nh.m_off, nh.m_s};
+    segs[n] = nh.segs;
+    std::copy(nh.class_init.begin(), nh.class_init.end(),
+              class_init.begin() + static_cast<size_t>(n) * (ctx->T + 1));
+    std::fill(net_of.begin() + nh.node_off, net_of.begin() + nh.node_off + nh.net.n_nodes, n);
+  }
+  HIP_OR_FAIL(ctx, hipSetDevice(p.device));
+  HIP_OR_FAIL(ctx, hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
+  const size_t NW = static_cast<size_t>(ctx->N) * ctx->W;
+  const size_t T = static_cast<size_t>(ctx->T);
+  const size_t G = static_cast<size_t>(ctx->n_net) * ctx->M;
+  int rc;
+  if ((rc = upload(ctx, &ctx->d_entries, reinterpret_cast<const int4 *>(entries.data()),
+                   entries.size())) ||
+      (rc = upload(ctx, &ctx->d_rowptr, rowptr.data(), rowptr.size())) ||
+      (rc = upload(ctx, &ctx->d_nets, nd.data(), nd.size())) ||
+      (rc = upload(ctx, &ctx->d_segs, segs.data(), segs.size())) ||
+      (rc = upload(ctx, &ctx->d_class_init, class_init.data(), class_init.size())) ||
+      (rc = upload(ctx, &ctx->d_net_of, net_of.data(), net_of.size())) ||
+      (rc = (ctx->any_cut ? upload(ctx, &ctx->d_rank_limit, limits.data(), limits.size())
+                          : REPLAY_OK)) ||
+      (rc = dev_alloc(ctx, &ctx->d_S, NW)) || (rc = dev_alloc(ctx, &ctx->d_I[0], NW)) ||
+      (rc = dev_alloc(ctx, &ctx->d_I[1], NW)) ||
+      (rc = dev_alloc(ctx, &ctx->d_X, NW * static_cast<size_t>(ctx->depth))) ||
+      (rc = dev_alloc(ctx, &ctx->d_Xinit, NW)) ||
+      (rc = dev_alloc(ctx, &ctx->d_entry, static_cast<size_t>(ctx->N) * ctx->Mp)) ||
+      (rc = dev_alloc(ctx, &ctx->d_inf, (T + 1) * G)) ||
+      (rc = dev_alloc(ctx, &ctx->d_P, (T + 2) * G)) ||
+      (rc = dev_alloc(ctx, &ctx->d_init_count, G)) ||
+      (rc = dev_alloc(ctx, &ctx->d_counts, (T + 1) * G * 4)))
+    return rc;
+  ctx->variant = ctx->W <= 4 ? REPLAY_V_NODE4 : REPLAY_V_NODE16;
+  rc = do_reset(ctx);
+  if (rc != REPLAY_OK) return rc;
+  HIP_OR_FAIL(ctx, hipStreamSynchronize(ctx->stream));
+  if (p.flags & REPLAY_F_AUTOTUNE) {
+    int32_t v;
+    rc = replay_autotune(ctx, std::min<int32_t>(ctx->T, 32), &v);
+    if (rc != REPLAY_OK) return rc;
+  }
+  return REPLAY_OK;
+}
+
+int open_common(replay_ctx *ctx, replay_ctx **out) {
+  int rc = finish_open(ctx);
   if (rc != REPLAY_OK) {
     g_open_error = ctx->err;
     release(ctx);
@@ -778,6 +1060,46 @@ int open_common(replay_ctx *ctx, const replay_params *p, replay_ctx **out) {
     return rc;
   }
   *out = ctx;
+  return REPLAY_OK;
+}
+
+NetHost single_net(const replay_params *p) {
+  NetHost nh;
+  nh.p0 = p->initial_infected;
+  nh.beta = p->infect_prob;
+  nh.Ed = p->exposed_duration;
+  nh.Id = p->infectious_duration;
+  nh.Rd = p->resistant_duration;
+  nh.stream = p->stream;
+  return nh;
+}
+
+// Captures the step and counts kernels of a full run (iterations 0..T-1
+// from a reset state) into one executable graph.
+int build_graph(replay_ctx *ctx) {
+  const int32_t saved_cur = ctx->cur, saved_done = ctx->steps_done, saved_counts = ctx->counts_done;
+  ctx->cur = 0;
+  ctx->steps_done = 0;
+  ctx->counts_done = -1;
+  hipGraph_t g = nullptr;
+  HIP_OR_FAIL(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+  int rc = enqueue_steps(ctx, ctx->T);
+  if (rc == REPLAY_OK) rc = enqueue_counts(ctx);
+  hipError_t e = hipStreamEndCapture(ctx->stream, &g);
+  ctx->cur = saved_cur;
+  ctx->steps_done = saved_done;
+  ctx->counts_done = saved_counts;
+  if (rc != REPLAY_OK) {
+    if (g) hipGraphDestroy(g);
+    return rc;
+  }
+  if (e != hipSuccess) return fail(ctx, REPLAY_E_DEVICE, std::string("capture: ") + hipGetErrorString(e));
+  e = hipGraphInstantiate(&ctx->graph, g, nullptr, nullptr, 0);
+  hipGraphDestroy(g);
+  if (e != hipSuccess) {
+    ctx->graph = nullptr;
+    return fail(ctx, REPLAY_E_DEVICE, std::string("instantiate: ") + hipGetErrorString(e));
+  }
   return REPLAY_OK;
 }
 
@@ -801,36 +1123,86 @@ int32_t replay_open(const char *graph_path, const replay_params *p, replay_ctx *
   if (!out || !graph_path) return fail(nullptr, REPLAY_E_ARG, "null argument");
   *out = nullptr;
   int rc = check_params(p);
+  if (rc == REPLAY_OK)
+    rc = check_durations(p->exposed_duration, p->infectious_duration, p->resistant_duration);
   if (rc != REPLAY_OK) return rc;
   replay_ctx *ctx = new (std::nothrow) replay_ctx();
   if (!ctx) return fail(nullptr, REPLAY_E_NOMEM, "out of host memory");
   ctx->p = *p;
+  ctx->nets.push_back(single_net(p));
   std::string err;
-  rc = rp::parse_file(graph_path, ctx->net, err);
+  rc = rp::parse_file(graph_path, ctx->nets[0].net, err);
   if (rc != REPLAY_OK) {
     delete ctx;
     return fail(nullptr, rc, err);
   }
-  return open_common(ctx, p, out);
+  return open_common(ctx, out);
 }
 
 int32_t replay_open_contacts(int32_t n_node_list, const int64_t *node_ids, int64_t n_contacts,
                              const int32_t *src, const int32_t *tgt, const int64_t *start,
                              const int64_t *end, const replay_params *p, replay_ctx **out) {
   if (!out) return fail(nullptr, REPLAY_E_ARG, "null argument");
+  replay_network n;
+  std::memset(&n, 0, sizeof n);
+  n.n_node_list = n_node_list;
+  n.node_ids = node_ids;
+  n.n_contacts = n_contacts;
+  n.src = src;
+  n.tgt = tgt;
+  n.start = start;
+  n.end = end;
+  if (p) {
+    n.initial_infected = p->initial_infected;
+    n.infect_prob = p->infect_prob;
+    n.exposed_duration = p->exposed_duration;
+    n.infectious_duration = p->infectious_duration;
+    n.resistant_duration = p->resistant_duration;
+    n.stream = p->stream;
+  }
+  return replay_open_batch(1, &n, p, out);
+}
+
+int32_t replay_open_batch(int32_t n_networks, const replay_network *nets, const replay_params *p,
+                          replay_ctx **out) {
+  if (!out || !nets || n_networks <= 0) return fail(nullptr, REPLAY_E_ARG, "null argument");
   *out = nullptr;
   int rc = check_params(p);
   if (rc != REPLAY_OK) return rc;
+  for (int32_t n = 0; n < n_networks; ++n) {
+    rc = check_durations(nets[n].exposed_duration, nets[n].infectious_duration,
+                         nets[n].resistant_duration);
+    if (rc != REPLAY_OK) return rc;
+  }
   replay_ctx *ctx = new (std::nothrow) replay_ctx();
   if (!ctx) return fail(nullptr, REPLAY_E_NOMEM, "out of host memory");
   ctx->p = *p;
-  std::string err;
-  rc = rp::from_contacts(n_node_list, node_ids, n_contacts, src, tgt, start, end, ctx->net, err);
-  if (rc != REPLAY_OK) {
-    delete ctx;
-    return fail(nullptr, rc, err);
+  ctx->nets.resize(static_cast<size_t>(n_networks));
+  for (int32_t n = 0; n < n_networks; ++n) {
+    const replay_network &d = nets[n];
+    NetHost &nh = ctx->nets[n];
+    nh.p0 = d.initial_infected;
+    nh.beta = d.infect_prob;
+    nh.Ed = d.exposed_duration;
+    nh.Id = d.infectious_duration;
+    nh.Rd = d.resistant_duration;
+    nh.stream = d.stream;
+    std::string err;
+    rc = rp::from_contacts(d.n_node_list, d.node_ids, d.n_contacts, d.src, d.tgt, d.start, d.end,
+                           nh.net, err);
+    if (rc != REPLAY_OK) {
+      delete ctx;
+      return fail(nullptr, rc, "network " + std::to_string(n) + ": " + err);
+    }
   }
-  return open_common(ctx, p, out);
+  return open_common(ctx, out);
+}
+
+int32_t replay_get_network_offsets(replay_ctx *ctx, int32_t *node_off, int32_t n) {
+  if (!ctx || !node_off || n < ctx->n_net + 1) return REPLAY_E_ARG;
+  for (int32_t k = 0; k < ctx->n_net; ++k) node_off[k] = ctx->nets[k].node_off;
+  node_off[ctx->n_net] = ctx->N;
+  return REPLAY_OK;
 }
 
 int32_t replay_get_info(replay_ctx *ctx, replay_info *out) {
@@ -838,32 +1210,36 @@ int32_t replay_get_info(replay_ctx *ctx, replay_info *out) {
   std::memset(out, 0, sizeof *out);
   out->n_nodes = ctx->N;
   out->M = ctx->M;
-  out->n_contacts = static_cast<int64_t>(ctx->net.contacts.size());
-  out->nlists = ctx->net.nlists;
-  out->iterations = ctx->p.iterations;
-  out->n_windows = static_cast<int32_t>(ctx->plan.windows.size());
-  out->n_epochs = static_cast<int32_t>(ctx->slices.epoch_t0.size());
-  out->windows_per_epoch = ctx->slices.windows_per_epoch;
-  out->n_entries = static_cast<int64_t>(ctx->slices.entries.size());
-  out->slice_bytes = static_cast<int64_t>(ctx->slices.entries.size() * sizeof(rp::Entry) +
-                                          ctx->slices.rowptr.size() * sizeof(int32_t));
+  out->n_contacts = ctx->n_contacts;
+  out->nlists = ctx->nlists;
+  out->iterations = ctx->T;
+  out->n_windows = static_cast<int32_t>(ctx->win_ws.size());
+  out->n_epochs = static_cast<int32_t>(ctx->epoch_t0.size());
+  out->windows_per_epoch = ctx->windows_per_epoch;
+  out->n_entries = ctx->n_entries;
+  out->slice_bytes = ctx->n_entries * static_cast<int64_t>(sizeof(rp::Entry)) +
+                     static_cast<int64_t>(ctx->epoch_t0.size()) * (ctx->N + 1) * 4;
   const int64_t NW = static_cast<int64_t>(ctx->N) * ctx->W;
-  out->state_bytes = NW * 8 * (4 + ctx->tp.depth) + static_cast<int64_t>(ctx->N) * ctx->Mp * 4;
+  out->state_bytes = NW * 8 * (4 + ctx->depth) + static_cast<int64_t>(ctx->N) * ctx->Mp * 4;
   out->steps_done = ctx->steps_done;
   out->truncated_windows = ctx->truncated;
   // per word: S read + write, own I read, I_next write, X_k write, and one
-  // ring read per flip offset kept in the ring; plus the slice epoch's
-  // entries and row pointers, and the per-replica prefix update
+  // ring read per flip offset kept in the ring (the first network's offsets);
+  // plus the slice epoch's entries and row pointers, and the prefix update
   int ring_reads = 0;
-  for (int32_t n : {ctx->tp.n_on, ctx->tp.n_off, ctx->tp.n_s})
-    ring_reads += (n != kNever && n >= 2 && n <= ctx->p.iterations) ? 1 : 0;
-  const double ent = ctx->slices.epoch_t0.empty()
-                         ? 0.0
-                         : static_cast<double>(ctx->slices.entries.size()) /
-                               static_cast<double>(ctx->slices.epoch_t0.size());
+  const NetHost &n0 = ctx->nets[0];
+  for (int32_t n : {n0.n_on, n0.n_off, n0.n_s})
+    ring_reads += (n != kNever && n >= 2 && n <= ctx->T) ? 1 : 0;
+  const double ent = ctx->epoch_t0.empty() ? 0.0
+                                           : static_cast<double>(ctx->n_entries) /
+                                                 static_cast<double>(ctx->epoch_t0.size());
   out->step_bytes = NW * 8 * (5 + ring_reads) + static_cast<int64_t>(16.0 * ent) +
-                    4 * (static_cast<int64_t>(ctx->N) + 1) + 12 * static_cast<int64_t>(ctx->M);
-  out->ring_depth = ctx->tp.depth;
+                    4 * (static_cast<int64_t>(ctx->N) + 1) +
+                    12 * static_cast<int64_t>(ctx->n_net) * ctx->M;
+  out->ring_depth = ctx->depth;
+  out->n_networks = ctx->n_net;
+  out->variant = ctx->variant;
+  out->graph_ready = ctx->graph != nullptr;
   return REPLAY_OK;
 }
 
@@ -875,22 +1251,23 @@ int32_t replay_reset(replay_ctx *ctx) {
 
 int32_t replay_run(replay_ctx *ctx, int32_t steps) {
   if (!ctx) return REPLAY_E_ARG;
-  if (steps < 0 || ctx->steps_done + steps > ctx->p.iterations)
+  if (steps < 0 || ctx->steps_done + steps > ctx->T)
     return fail(ctx, REPLAY_E_ARG, "steps exceed the planned iterations");
   HIP_OR_FAIL(ctx, hipSetDevice(ctx->p.device));
-  // threads cover every word and, for the running prefix, every replica
-  const int64_t NW = static_cast<int64_t>(ctx->N) * ctx->W;
-  const int64_t cover = std::max<int64_t>(NW, ctx->M);
-  const dim3 grid(static_cast<unsigned>((cover + kBlock - 1) / kBlock));
-  for (int32_t s = 0; s < steps; ++s) {
-    const int32_t k = ctx->steps_done;
-    StepArgs a = step_args(ctx, k);
-    const bool int_mode = ctx->plan.windows[ctx->plan.iter_window[k]].int_mode != 0;
-    launch_step(a, int_mode, a.uniforms != nullptr, grid, ctx->stream);
-    HIP_OR_FAIL(ctx, hipGetLastError());
-    ctx->cur ^= 1;
-    ctx->steps_done = k + 1;
+  if ((ctx->p.flags & REPLAY_F_GRAPH) && ctx->steps_done == 0 && steps == ctx->T && steps > 0 &&
+      !ctx->d_uniforms) {
+    if (!ctx->graph) {
+      int rc = build_graph(ctx);
+      if (rc != REPLAY_OK) return rc;
+    }
+    HIP_OR_FAIL(ctx, hipGraphLaunch(ctx->graph, ctx->stream));
+    ctx->steps_done = ctx->T;
+    ctx->cur = ctx->T & 1;
+    ctx->counts_done = ctx->T;
+    return REPLAY_OK;
   }
+  int rc = enqueue_steps(ctx, steps);
+  if (rc != REPLAY_OK) return rc;
   return enqueue_counts(ctx);
 }
 
@@ -914,6 +1291,46 @@ int32_t replay_run_timed(replay_ctx *ctx, int32_t steps, double *elapsed_ms) {
   return rc;
 }
 
+int32_t replay_autotune(replay_ctx *ctx, int32_t steps, int32_t *variant) {
+  if (!ctx || !variant) return REPLAY_E_ARG;
+  steps = std::min(steps, ctx->T);
+  HIP_OR_FAIL(ctx, hipSetDevice(ctx->p.device));
+  const int candidates[] = {REPLAY_V_WORD, REPLAY_V_NODE4, REPLAY_V_NODE16};
+  int best = ctx->variant;
+  double best_ms = 1e30;
+  for (int v : candidates) {
+    if (!variant_applies(ctx, v) || steps <= 0) continue;
+    ctx->variant = v;
+    int rc = do_reset(ctx);
+    if (rc != REPLAY_OK) return rc;
+    double ms = 0.0;
+    rc = replay_run_timed(ctx, steps, &ms);
+    if (rc != REPLAY_OK) return rc;
+    if (ms < best_ms) {
+      best_ms = ms;
+      best = v;
+    }
+  }
+  ctx->variant = best;
+  if (ctx->graph) {  // captured with another variant
+    hipGraphExecDestroy(ctx->graph);
+    ctx->graph = nullptr;
+  }
+  *variant = best;
+  return do_reset(ctx);
+}
+
+int32_t replay_set_variant(replay_ctx *ctx, int32_t variant) {
+  if (!ctx || !variant_applies(ctx, variant)) return REPLAY_E_ARG;
+  if (ctx->variant != variant && ctx->graph) {
+    HIP_OR_FAIL(ctx, hipStreamSynchronize(ctx->stream));
+    hipGraphExecDestroy(ctx->graph);
+    ctx->graph = nullptr;
+  }
+  ctx->variant = variant;
+  return REPLAY_OK;
+}
+
 int32_t replay_sync(replay_ctx *ctx) {
   if (!ctx) return REPLAY_E_ARG;
   HIP_OR_FAIL(ctx, hipStreamSynchronize(ctx->stream));
@@ -922,7 +1339,8 @@ int32_t replay_sync(replay_ctx *ctx) {
 
 int32_t replay_get_counts(replay_ctx *ctx, int32_t *out, int64_t n_out) {
   if (!ctx || !out) return REPLAY_E_ARG;
-  const int64_t n = static_cast<int64_t>(ctx->steps_done + 1) * ctx->M * 4;
+  const int64_t G = static_cast<int64_t>(ctx->n_net) * ctx->M;
+  const int64_t n = static_cast<int64_t>(ctx->steps_done + 1) * G * 4;
   if (n_out < n) return fail(ctx, REPLAY_E_ARG, "counts buffer too small");
   HIP_OR_FAIL(ctx, hipSetDevice(ctx->p.device));
   int rc = enqueue_counts(ctx);
@@ -930,7 +1348,10 @@ int32_t replay_get_counts(replay_ctx *ctx, int32_t *out, int64_t n_out) {
   HIP_OR_FAIL(ctx, hipMemcpyAsync(out, ctx->d_counts, static_cast<size_t>(n) * sizeof(int32_t),
                                   hipMemcpyDeviceToHost, ctx->stream));
   HIP_OR_FAIL(ctx, hipStreamSynchronize(ctx->stream));
-  for (int64_t b = 0; b < n; b += 4) out[b + 3] = ctx->N - out[b] - out[b + 1] - out[b + 2];
+  for (int64_t b = 0; b < n; b += 4) {
+    const int32_t net = static_cast<int32_t>((b / 4 / ctx->M) % ctx->n_net);
+    out[b + 3] = ctx->nets[net].net.n_nodes - out[b] - out[b + 1] - out[b + 2];
+  }
   return REPLAY_OK;
 }
 
@@ -944,24 +1365,26 @@ int32_t replay_get_state(replay_ctx *ctx, int32_t *out, int64_t n_out) {
                                   hipMemcpyDeviceToHost, ctx->stream));
   HIP_OR_FAIL(ctx, hipStreamSynchronize(ctx->stream));
   const int32_t k = ctx->steps_done;
-  const auto &ln = ctx->tp.lut_new, &li = ctx->tp.lut_init;
-  for (int32_t i = 0; i < ctx->N; ++i)
-    for (int32_t r = 0; r < ctx->M; ++r) {
-      const int32_t s = ent[static_cast<size_t>(i) * ctx->Mp + r];
-      int32_t c = 0;
-      if (s == 0)
-        c = li[k];
-      else if (s > 0)
-        c = ln[k - (s - 1) / 2];
-      out[static_cast<size_t>(r) * ctx->N + i] = c;
-    }
+  for (const NetHost &nh : ctx->nets) {
+    const auto &ln = nh.lut_new, &li = nh.lut_init;
+    for (int32_t i = nh.node_off; i < nh.node_off + nh.net.n_nodes; ++i)
+      for (int32_t r = 0; r < ctx->M; ++r) {
+        const int32_t s = ent[static_cast<size_t>(i) * ctx->Mp + r];
+        int32_t c = 0;
+        if (s == 0)
+          c = li[k];
+        else if (s > 0)
+          c = ln[k - (s - 1) / 2];
+        out[static_cast<size_t>(r) * ctx->N + i] = c;
+      }
+  }
   return REPLAY_OK;
 }
 
 int64_t replay_iteration_time(replay_ctx *ctx, int32_t k) {
   if (!ctx) return 0;
-  if (k < 0 || k >= static_cast<int32_t>(ctx->plan.iter_time.size())) return ctx->p.start_time;
-  return ctx->plan.iter_time[k];
+  if (k < 0 || k >= static_cast<int32_t>(ctx->iter_time.size())) return ctx->p.start_time;
+  return ctx->iter_time[k];
 }
 
 int32_t replay_set_uniforms(replay_ctx *ctx, const float *u, int64_t n) {
@@ -973,7 +1396,7 @@ int32_t replay_set_uniforms(replay_ctx *ctx, const float *u, int64_t n) {
     ctx->d_uniforms = nullptr;
   }
   if (u) {
-    const int64_t need = static_cast<int64_t>(ctx->p.iterations + 1) * ctx->M * ctx->N;
+    const int64_t need = static_cast<int64_t>(ctx->T + 1) * ctx->M * ctx->N;
     if (n != need) return fail(ctx, REPLAY_E_ARG, "uniform stream has the wrong length");
     int rc = dev_alloc(ctx, &ctx->d_uniforms, static_cast<size_t>(n));
     if (rc) return rc;
@@ -985,7 +1408,7 @@ int32_t replay_set_uniforms(replay_ctx *ctx, const float *u, int64_t n) {
 
 int32_t replay_get_slice(replay_ctx *ctx, int32_t k, int32_t *rowptr, int32_t *colind, float *val,
                          int64_t cap, int64_t *nnz) {
-  if (!ctx || !rowptr || !nnz || k < 0 || k >= ctx->p.iterations) return REPLAY_E_ARG;
+  if (!ctx || !rowptr || !nnz || k < 0 || k >= ctx->T) return REPLAY_E_ARG;
   HIP_OR_FAIL(ctx, hipSetDevice(ctx->p.device));
   StepArgs a = step_args(ctx, k);
   int32_t *d_rp = nullptr, *d_cnt = nullptr, *d_col = nullptr;
@@ -1031,7 +1454,7 @@ int32_t replay_get_slice(replay_ctx *ctx, int32_t k, int32_t *rowptr, int32_t *c
 }
 
 int32_t replay_get_exposure(replay_ctx *ctx, int32_t k, float *out, int64_t n_out) {
-  if (!ctx || !out || k < 0 || k >= ctx->p.iterations) return REPLAY_E_ARG;
+  if (!ctx || !out || k < 0 || k >= ctx->T) return REPLAY_E_ARG;
   const int64_t n = static_cast<int64_t>(ctx->N) * ctx->M;
   if (n_out < n) return fail(ctx, REPLAY_E_ARG, "exposure buffer too small");
   HIP_OR_FAIL(ctx, hipSetDevice(ctx->p.device));
@@ -1040,7 +1463,7 @@ int32_t replay_get_exposure(replay_ctx *ctx, int32_t k, float *out, int64_t n_ou
   int rc = dev_alloc(ctx, &d_out, static_cast<size_t>(n));
   if (rc) return rc;
   dim3 grid((ctx->N + kWavesPerBlock - 1) / kWavesPerBlock, ctx->W);
-  if (ctx->plan.windows[ctx->plan.iter_window[k]].int_mode)
+  if (ctx->win_int[ctx->iter_window[k]])
     hipLaunchKernelGGL(exposure_kernel<true>, grid, dim3(kBlock), 0, ctx->stream, a, d_out);
   else
     hipLaunchKernelGGL(exposure_kernel<false>, grid, dim3(kBlock), 0, ctx->stream, a, d_out);

@@ -151,6 +151,61 @@ def config2_dense(seed: int = 0, n_nodes: int = 50000, n_contacts: int = 40_000_
                     start.astype(np.int64), (start + dur).astype(np.int64), p)
 
 
+@dataclass
+class BatchWorkload:
+    name: str
+    networks: list          # list of replay_amd.Network
+    params: Params          # shared: M, step, iterations, windows, start, seed
+    notes: dict = field(default_factory=dict)
+
+    @property
+    def n_nodes(self) -> int:
+        return sum(n.n_nodes for n in self.networks)
+
+    @property
+    def n_contacts(self) -> int:
+        return sum(int(n.src.size) for n in self.networks)
+
+
+def config3_batched(seed: int = 0, n_networks: int = 64, M: int = 256, iterations: int = 168,
+                    n_min: int = 1000, n_max: int = 40000, deg_min: float = 20.0,
+                    deg_max: float = 200.0, mean_days: float = 7.0) -> BatchWorkload:
+    """Batched uneven workload: 64 networks with node counts log-uniform in
+    1k-40k and 20-200 simultaneous neighbours, 256 replicas each, one-hour
+    steps over a week, each with (beta, 1/sigma, 1/gamma) log-uniform within
+    +-50% of the village configuration's (0.03, 48 h, 120 h).
+
+    Contacts are uniform random pairs; starts are uniform over
+    [t0 - mean, t0 + horizon) and durations Exp(mean) capped at 4x the mean,
+    so about N * degree / 2 contacts are active at any time."""
+    from . import Network
+    rng = np.random.default_rng(seed)
+    H = iterations * HOUR
+    D = mean_days * DAY
+    nets = []
+    logu = lambda lo, hi: float(np.exp(rng.uniform(np.log(lo), np.log(hi))))
+    for k in range(n_networks):
+        n = int(round(logu(n_min, n_max)))
+        deg = logu(deg_min, deg_max)
+        active = n * deg / 2.0
+        n_c = int(round(active * (H + D) / D))
+        a, b = _pairs_uniform(rng, n, n_c)
+        start = T0 + rng.integers(-int(D), H, size=n_c)
+        dur = np.minimum(np.rint(rng.exponential(D, size=n_c)), 4 * D).astype(np.int64)
+        dur = np.maximum(dur, 1)
+        start = np.maximum(start, 0)
+        nets.append(Network(n_nodes=n, src=a.astype(np.int32), tgt=b.astype(np.int32),
+                            start=start.astype(np.int64), end=(start + dur).astype(np.int64),
+                            initial_infected=0.001,
+                            infect_prob=0.03 * logu(0.5, 1.5),
+                            exposed_duration=int(48 * HOUR * logu(0.5, 1.5)),
+                            infectious_duration=int(120 * HOUR * logu(0.5, 1.5)),
+                            resistant_duration=int(H + DAY), stream=k))
+    p = Params(M=M, step_size=float(HOUR), iterations=iterations, start_time=T0,
+               static_network_duration=HOUR, seed=0)
+    return BatchWorkload("config3_batched_64nets", nets, p)
+
+
 def scaled(w: Workload, **kw) -> Workload:
     return replace(w, params=replace(w.params, **kw))
 

@@ -1,0 +1,118 @@
+"""Batched form (several networks in one context, one step launch for all)
+against the plain form (one context per network) and the oracle, plus
+hipGraph replays and kernel-variant agreement.  Needs an MI355X."""
+import numpy as np
+import pytest
+
+import replay_amd
+from replay_amd import Network, Params, Simulation, workloads
+
+from test_gpu_parity import _random_case, _to_oracle_params
+
+pytestmark = pytest.mark.gpu
+
+
+def _nets(seed, sizes, long_frac=0.3):
+    rng = np.random.default_rng(seed)
+    out = []
+    for k, n in enumerate(sizes):
+        _, a, b, s, e = _random_case(seed * 31 + k, n=n, e=n * 12, long_frac=long_frac)
+        out.append(Network(n_nodes=n, src=a, tgt=b, start=s, end=e,
+                           initial_infected=float(rng.uniform(0.02, 0.2)),
+                           infect_prob=float(rng.uniform(0.1, 0.6)),
+                           exposed_duration=int(rng.integers(0, 4)) * 1800,
+                           infectious_duration=int(rng.integers(2, 8)) * 1800,
+                           resistant_duration=int(rng.choice([0, 3600, 10 ** 6])),
+                           stream=100 + k))
+    return out
+
+
+def _plain_params(shared: Params, nw: Network) -> Params:
+    return Params(M=shared.M, step_size=shared.step_size, iterations=shared.iterations,
+                  initial_infected=nw.initial_infected, infect_prob=nw.infect_prob,
+                  exposed_duration=nw.exposed_duration,
+                  infectious_duration=nw.infectious_duration,
+                  resistant_duration=nw.resistant_duration, start_time=shared.start_time,
+                  static_network_duration=shared.static_network_duration, seed=shared.seed,
+                  stream=nw.stream, windows_per_epoch=shared.windows_per_epoch)
+
+
+@pytest.mark.parametrize("M", [64, 200, 1024])
+@pytest.mark.parametrize("step,D", [(600.0, 600), (1000.0, 3600)])
+def test_batch_equals_plain_and_oracle(oracle, M, step, D):
+    nets = _nets(M + int(step), [37, 80, 5, 150])
+    shared = Params(M=M, step_size=step, static_network_duration=D, iterations=90, seed=7)
+    with Simulation.from_networks(nets, shared) as b:
+        assert b.n_networks == 4
+        b.run()
+        bc = b.counts()
+        bs = b.state()
+        off = b.node_offsets
+    N = int(off[-1])
+    for k, nw in enumerate(nets):
+        pp = _plain_params(shared, nw)
+        with Simulation.from_contacts(nw.n_nodes, nw.src, nw.tgt, nw.start, nw.end, pp) as s:
+            s.run()
+            np.testing.assert_array_equal(s.counts(), bc[:, k])
+            st = s.state().reshape(M, nw.n_nodes)
+            np.testing.assert_array_equal(st, bs.reshape(M, N)[:, off[k]:off[k + 1]])
+        g = oracle.OracleGraph.from_contacts(nw.n_nodes, nw.src, nw.tgt, nw.start, nw.end)
+        op = _to_oracle_params(oracle, pp)
+        op.stream = nw.stream
+        counts, _, _ = g.simulate(op)
+        np.testing.assert_array_equal(counts, bc[:, k])
+
+
+def test_graph_replays_match_eager():
+    nets = _nets(3, [60, 90, 33])
+    shared = Params(M=128, step_size=900.0, static_network_duration=1800, iterations=80, seed=1)
+    with Simulation.from_networks(nets, shared) as eager:
+        eager.run()
+        ref = eager.counts()
+        ref_state = eager.state()
+    gp = Params(**{**shared.__dict__, "flags": replay_amd.F_GRAPH})
+    with Simulation.from_networks(nets, gp) as g:
+        for rep in range(3):
+            g.reset()
+            g.run()
+            assert g.info()["graph_ready"] == 1
+            np.testing.assert_array_equal(g.counts(), ref)
+            np.testing.assert_array_equal(g.state(), ref_state)
+            # eager work between replays
+            g.reset()
+            g.run(10)
+            g.sync()
+
+
+@pytest.mark.parametrize("M", [256, 1000])
+def test_variants_agree(M):
+    nets = _nets(5, [120, 70])
+    shared = Params(M=M, step_size=600.0, static_network_duration=600, iterations=60, seed=2)
+    results = []
+    with Simulation.from_networks(nets, shared) as sim:
+        for v in (replay_amd.V_WORD, replay_amd.V_NODE4, replay_amd.V_NODE16):
+            sim.set_variant(v)
+            sim.reset()
+            sim.run()
+            results.append((sim.counts(), sim.state()))
+        best = sim.autotune(20)
+        assert best in (replay_amd.V_WORD, replay_amd.V_NODE4, replay_amd.V_NODE16)
+        sim.run()
+        results.append((sim.counts(), sim.state()))
+    for c, s in results[1:]:
+        np.testing.assert_array_equal(c, results[0][0])
+        np.testing.assert_array_equal(s, results[0][1])
+
+
+def test_config3_reduced_batch_vs_plain():
+    w = workloads.config3_batched(n_networks=6, n_min=300, n_max=3000, iterations=48,
+                                  mean_days=2.0)
+    with Simulation.from_networks(w.networks, w.params) as b:
+        b.run()
+        bc = b.counts()
+    for k, nw in enumerate(w.networks):
+        with Simulation.from_contacts(nw.n_nodes, nw.src, nw.tgt, nw.start, nw.end,
+                                      _plain_params(w.params, nw)) as s:
+            s.run()
+            np.testing.assert_array_equal(s.counts(), bc[:, k])
+    assert bc[-1, :, :, 1:3].sum() > 0
