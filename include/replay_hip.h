@@ -170,9 +170,11 @@ int32_t replay_get_info(replay_ctx *ctx, replay_info *out);
 int32_t replay_reset(replay_ctx *ctx);
 
 /* Enqueue `steps` more iterations of the step loop (cu:859-1150) on the
-   context's stream; returns once they are enqueued.  Each iteration runs
-   mask -> exposure -> probability -> draw -> countdown update -> counts in
-   one fused kernel. */
+   context's stream; returns once they are enqueued.  Each iteration is two
+   kernels: exposure -> probability -> draw (one wavefront per node), then
+   the timer update and the running prefix of infection counts (one thread
+   per 64-replica word).  SEIR counts are reduced from those prefixes on the
+   device when they are read. */
 int32_t replay_run(replay_ctx *ctx, int32_t steps);
 
 /* replay_run, bracketed by HIP events recorded on the context's stream;

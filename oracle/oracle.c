@@ -757,16 +757,20 @@ uint32_t or_philox_word(uint64_t seed, uint32_t c0, uint32_t c1, uint32_t c2,
 }
 
 /* Uniform for (slot, node, replica): slot 0 = initial draw, slot k+1 = draw
-   of iteration k.  Counter = {node, replica, slot, stream}, key = seed; the top
-   24 bits of the first output word scaled by 2^-24, so u is in [0, 1).
-   An injected stream is indexed [slot][replica][node]. */
+   of iteration k.  One Philox4x32-10 block serves four replicas: replica r
+   takes output word (r >> 6) & 3 of counter {node, (r >> 8) * 64 + (r & 63),
+   slot, stream} under key = seed; its top 24 bits scaled by 2^-24 give u in
+   [0, 1).  An injected stream is indexed [slot][replica][node]. */
 float or_uniform(const or_params *p, int slot, int node, int replica, int n_nodes) {
   if (p->uniforms)
     return p->uniforms[((size_t)slot * (size_t)p->M + (size_t)replica) * (size_t)n_nodes +
                        (size_t)node];
-  uint32_t w = or_philox_word(p->seed, (uint32_t)node, (uint32_t)replica,
-                              (uint32_t)slot, p->stream);
-  return (float)(w >> 8) * 0x1p-24f;
+  const uint32_t r = (uint32_t)replica;
+  uint32_t ctr[4] = {(uint32_t)node, ((r >> 8) << 6) | (r & 63u), (uint32_t)slot, p->stream};
+  uint32_t key[2] = {(uint32_t)p->seed, (uint32_t)(p->seed >> 32)};
+  uint32_t out[4];
+  or_philox4x32_10(ctr, key, out);
+  return (float)(out[(r >> 6) & 3u] >> 8) * 0x1p-24f;
 }
 
 /* ------------------------------------------------------------------------ */

@@ -34,10 +34,40 @@ __device__ __forceinline__ uint32_t philox_word0(uint64_t seed, uint32_t c0, uin
   return c0;
 }
 
-// Uniform in [0, 1): top 24 bits of Philox(counter = {node, sim, slot, stream}).
+// All four output words of Philox4x32-10.
+__device__ __forceinline__ void philox4(uint64_t seed, uint32_t c0, uint32_t c1, uint32_t c2,
+                                        uint32_t c3, uint32_t (&out)[4]) {
+  uint32_t k0 = static_cast<uint32_t>(seed), k1 = static_cast<uint32_t>(seed >> 32);
+#pragma unroll
+  for (int r = 0; r < 10; ++r) {
+    const uint32_t lo0 = 0xD2511F53u * c0, hi0 = __umulhi(0xD2511F53u, c0);
+    const uint32_t lo1 = 0xCD9E8D57u * c2, hi1 = __umulhi(0xCD9E8D57u, c2);
+    c0 = hi1 ^ c1 ^ k0;
+    c1 = lo1;
+    c2 = hi0 ^ c3 ^ k1;
+    c3 = lo0;
+    k0 += 0x9E3779B9u;
+    k1 += 0xBB67AE85u;
+  }
+  out[0] = c0;
+  out[1] = c1;
+  out[2] = c2;
+  out[3] = c3;
+}
+
+__device__ __forceinline__ float u24(uint32_t w) {
+  return static_cast<float>(w >> 8) * 0x1p-24f;
+}
+
+// Uniform in [0, 1) of replica `sim`: one Philox block serves four replicas
+// 64 apart, replica r taking word (r >> 6) & 3 of counter
+// {node, (r >> 8) * 64 + (r & 63), slot, stream}.
 __device__ __forceinline__ float philox_uniform(uint64_t seed, uint32_t node, uint32_t sim,
                                                 uint32_t slot, uint32_t stream) {
-  return static_cast<float>(philox_word0(seed, node, sim, slot, stream) >> 8) * 0x1p-24f;
+  uint32_t w[4];
+  philox4(seed, node, ((sim >> 8) << 6) | (sim & 63u), slot, stream, w);
+  const uint32_t k = (sim >> 6) & 3u;
+  return u24(k == 0 ? w[0] : k == 1 ? w[1] : k == 2 ? w[2] : w[3]);
 }
 
 __device__ __forceinline__ float rp_expf(float x) {

@@ -57,6 +57,7 @@ namespace {
 constexpr int kWave = 64;
 constexpr int kBlock = 256;
 constexpr int kWavesPerBlock = kBlock / kWave;
+constexpr unsigned kXcds = 8;  // MI355X accelerator complex dies
 constexpr int32_t kNever = INT32_MAX;
 constexpr int kMaxSegs = 16;
 
@@ -92,6 +93,8 @@ struct StepArgs {
   const NetDev *nets;
   const int32_t *net_of;      // node -> network
   const uint32_t *rank_limit; // [n_net] of this window, or null (no cap bites)
+  const float *p_tab;         // [n_net][tab_max+1]: P of c whole windows, or null
+  int32_t tab_max;
   uint64_t seed;
   int64_t NW;                 // N * W words
   int32_t N, M, W, Mp, n_net, depth;
@@ -156,23 +159,23 @@ __device__ __forceinline__ float row_exposure(int e0, int e1, const int4 *entrie
   }
 }
 
-// 64 x 64 bit-matrix transpose across a wavefront: lane j holds row j on
-// entry, lane r holds column r on exit (bit j of the result = bit r of row
-// j).  Six block-swap stages, largest blocks first: at stage s the rows
-// j (j & s == 0) and j + s exchange the s-bit column blocks that lie across
-// the diagonal of each 2s x 2s block.
-__device__ __forceinline__ uint64_t transpose64(uint64_t a, int lane) {
-  constexpr uint64_t kLow[6] = {0x00000000FFFFFFFFull, 0x0000FFFF0000FFFFull,
-                                0x00FF00FF00FF00FFull, 0x0F0F0F0F0F0F0F0Full,
-                                0x3333333333333333ull, 0x5555555555555555ull};
+// Column counts of a 64 x 64 bit matrix held one row per lane: on exit
+// lane r holds the number of rows with bit r set.  A butterfly over the lane
+// bits: at stage k, lanes differing in bit k exchange their words, each keeps
+// the fields (of width 2^k) whose column has its own bit k and adds the
+// partner's, so field widths double while the fields per lane halve.
+__device__ __forceinline__ uint32_t column_count(uint64_t x, int lane) {
+  constexpr uint64_t kLow[6] = {0x5555555555555555ull, 0x3333333333333333ull,
+                                0x0F0F0F0F0F0F0F0Full, 0x00FF00FF00FF00FFull,
+                                0x0000FFFF0000FFFFull, 0x00000000FFFFFFFFull};
 #pragma unroll
   for (int k = 0; k < 6; ++k) {
-    const int s = 32 >> k;
-    const uint64_t lo = kLow[k];
-    const uint64_t b = __shfl_xor(a, s);
-    a = (lane & s) ? (((b & ~lo) >> s) | (a & ~lo)) : ((a & lo) | ((b & lo) << s));
+    const int w = 1 << k;
+    const uint64_t y = __shfl_xor(x, w);
+    const int sh = (lane & w) ? w : 0;
+    x = ((x >> sh) & kLow[k]) + ((y >> sh) & kLow[k]);
   }
-  return a;
+  return static_cast<uint32_t>(x);
 }
 
 // Exposure of the words q0 .. q0+nq-1 (nq <= WT) of node i for one window.
@@ -186,13 +189,16 @@ __device__ __forceinline__ uint64_t transpose64(uint64_t a, int lane) {
 template <bool kIntMode, int WT>
 __device__ __forceinline__ void group_exposure(const StepArgs &a, int e0, int e1,
                                                uint32_t rank_limit, int q0, int nq, int lane,
-                                               float (&E)[WT]) {
+                                               const uint64_t (&ss)[WT], float (&E)[WT],
+                                               uint32_t (&cnt)[WT], uint32_t (&part)[WT]) {
   uint32_t iacc[WT];
   float facc[WT];
 #pragma unroll
   for (int q = 0; q < WT; ++q) {
     iacc[q] = 0;
     facc[q] = 0.0f;
+    cnt[q] = 0;
+    part[q] = 0;
   }
   // float mode: the column being merged, its weight and its words
   int cur_col = -1;
@@ -206,7 +212,9 @@ __device__ __forceinline__ void group_exposure(const StepArgs &a, int e0, int e1
     const bool ev = e < e1;
     int col = 0, ov = 0;
     if (ev) {
-      const int4 en = a.entries[e];
+      // streamed once per iteration: keep it out of the caches' way
+      typedef int v4i __attribute__((ext_vector_type(4)));
+      const v4i en = __builtin_nontemporal_load(reinterpret_cast<const v4i *>(a.entries + e));
       col = en.x;
       ov = min(en.z, a.we_rel) - max(en.y, a.ws_rel);
       if (!(ov > 0 && static_cast<uint32_t>(en.w) <= rank_limit)) ov = 0;
@@ -215,29 +223,44 @@ __device__ __forceinline__ void group_exposure(const StepArgs &a, int e0, int e1
     const uint64_t *mrow = a.I_cur + static_cast<size_t>(col) * a.W + q0;
 #pragma unroll
     for (int q = 0; q < WT; ++q) {
-      const uint64_t m = (ev && ov > 0 && q < nq) ? mrow[q] : 0ull;
+      const uint64_t m = (ev && ov > 0 && q < nq && ss[q] != 0ull) ? mrow[q] : 0ull;
       mlo[q] = static_cast<uint32_t>(m);
       mhi[q] = static_cast<uint32_t>(m >> 32);
     }
     if (kIntMode) {
-      // Transpose the 64 x 64 bit block (row j = entry j's word) so lane r
-      // holds bit r of every entry: contacts spanning the whole window add
-      // L * popcount, partial overlaps are added bit by bit.
+      // Whole-window contacts: count, per replica lane, the contacts whose
+      // neighbour word has that replica's bit set (column counts of the
+      // 64 x 64 block of gathered words); partial overlaps are added one
+      // contact at a time from the broadcast word.
       const int L = a.we_rel - a.ws_rel;
-      const uint64_t full = __ballot(ov == L);
-      const uint64_t part = __ballot(ov > 0 && ov != L);
+      const bool full = ov == L;
+      const uint64_t partial = __ballot(ov > 0 && ov != L);
 #pragma unroll
       for (int q = 0; q < WT; ++q) {
         const uint64_t m = (static_cast<uint64_t>(mhi[q]) << 32) | mlo[q];
-        if (q >= nq || __ballot(m != 0ull) == 0ull) continue;
-        const uint64_t t = transpose64(m, lane);
-        uint32_t add = static_cast<uint32_t>(L) * static_cast<uint32_t>(__popcll(t & full));
-        for (uint64_t pm = part & __ballot(m != 0ull); pm != 0ull; pm &= pm - 1) {
+        const uint64_t nz = __ballot(m != 0ull);
+        // words without susceptible replicas need no exposure
+        if (q >= nq || nz == 0ull || ss[q] == 0ull) continue;
+        // few contributing contacts: add them one at a time
+        const bool few = __popcll(nz & ~partial) <= 6;
+        const uint32_t c = (!few && (nz & ~partial)) ? column_count(full ? m : 0ull, lane) : 0u;
+        uint32_t add = 0;
+        for (uint64_t pm = few ? nz : (partial & nz); pm != 0ull; pm &= pm - 1) {
           const int j = __builtin_ctzll(pm);
+          const uint32_t lo = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(mlo[q]), j));
+          const uint32_t hi = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(mhi[q]), j));
+          const uint64_t mj = (static_cast<uint64_t>(hi) << 32) | lo;
           const uint32_t ovj = static_cast<uint32_t>(__builtin_amdgcn_readlane(ov, j));
-          add += static_cast<uint32_t>((t >> j) & 1ull) * ovj;
+          const uint32_t bit = static_cast<uint32_t>((mj >> lane) & 1ull);
+          if (ovj == static_cast<uint32_t>(L))
+            cnt[q] += bit;  // whole-window contact taken one at a time
+          else
+            add += bit * ovj;
+          iacc[q] += bit * ovj;
         }
-        iacc[q] += add;
+        cnt[q] += c;
+        part[q] += add;
+        iacc[q] += static_cast<uint32_t>(L) * c;
       }
       continue;
     }
@@ -282,20 +305,34 @@ __device__ __forceinline__ void group_exposure(const StepArgs &a, int e0, int e1
   for (int q = 0; q < WT; ++q) E[q] = kIntMode ? static_cast<float>(iacc[q]) : facc[q];
 }
 
-// Draw of one word's susceptible lanes given their exposure; records the
-// infections and returns their ballot.
+// Draw of one word (word Q of node ii) given its lanes' exposure; records the
+// infections and returns their ballot.  P_tab >= 0 is the tabulated
+// probability of this lane's exposure (a whole number of windows).  The
+// Philox block serving words 4g .. 4g+3 is computed once per lane and kept
+// in pw / pw_grp.
 template <bool kInjected>
 __device__ __forceinline__ uint64_t draw_word(const StepArgs &a, const NetDev &nd, int net,
-                                              int ii, int qq, int lane, uint64_t ss, float E) {
-  const int r = qq * kWave + lane;
+                                              int ii, int Q, int lane, uint64_t ss, float E,
+                                              float P_tab, uint32_t (&pw)[4], int &pw_grp) {
+  const int r = Q * kWave + lane;
   const bool sus = (ss >> lane) & 1ull;
   bool infect = false;
   // E == 0 gives P = 0 (or NaN), never above a uniform in [0, 1)
   if (sus && (kInjected || E > 0.0f)) {
-    const float P = 1.0f - rp::rp_expf(nd.log_base * (E / 3600.0f));
-    const float u = kInjected ? a.uniforms[static_cast<size_t>(r) * a.N + ii]
-                              : rp::philox_uniform(a.seed, static_cast<uint32_t>(ii - nd.node_off),
-                                                   static_cast<uint32_t>(r), a.slot, nd.stream);
+    const float P = P_tab >= 0.0f ? P_tab : 1.0f - rp::rp_expf(nd.log_base * (E / 3600.0f));
+    float u;
+    if (kInjected) {
+      u = a.uniforms[static_cast<size_t>(r) * a.N + ii];
+    } else {
+      if (pw_grp != (Q >> 2)) {
+        rp::philox4(a.seed, static_cast<uint32_t>(ii - nd.node_off),
+                    (static_cast<uint32_t>(Q >> 2) << 6) | static_cast<uint32_t>(lane), a.slot,
+                    nd.stream, pw);
+        pw_grp = Q >> 2;
+      }
+      const int k = Q & 3;
+      u = rp::u24(k == 0 ? pw[0] : k == 1 ? pw[1] : k == 2 ? pw[2] : pw[3]);
+    }
     infect = P > u;
   }
   if (infect) {
@@ -311,7 +348,14 @@ __device__ __forceinline__ uint64_t draw_word(const StepArgs &a, const NetDev &n
 template <bool kIntMode, bool kInjected, int WT>
 __global__ __launch_bounds__(kBlock) void expo_kernel(StepArgs a) {
   const int lane = threadIdx.x & (kWave - 1);
-  const int i = blockIdx.x * kWavesPerBlock + __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
+  // Workgroups are dealt to the 8 XCDs round-robin; remap so that XCD x
+  // takes the x-th contiguous eighth of the nodes.  Networks are disjoint
+  // row ranges, so an XCD then gathers neighbour words from a small part of
+  // the bitmap that stays in its own L2.  gridDim.x is a multiple of 8.
+  const unsigned per_xcd = gridDim.x / kXcds;
+  const unsigned lb = (blockIdx.x % kXcds) * per_xcd + blockIdx.x / kXcds;
+  const int i = static_cast<int>(lb) * kWavesPerBlock +
+                __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
   if (i >= a.N) return;
   uint64_t *X_k = a.X + static_cast<size_t>(a.k % a.depth) * a.NW + static_cast<size_t>(i) * a.W;
   const uint64_t *S = a.S + static_cast<size_t>(i) * a.W;
@@ -333,10 +377,21 @@ __global__ __launch_bounds__(kBlock) void expo_kernel(StepArgs a) {
     for (int q = 0; q < WT; ++q) x[q] = 0ull;
     if (any_sus != 0ull && (e1 > e0 || kInjected)) {
       float E[WT];
-      group_exposure<kIntMode, WT>(a, e0, e1, rlim, q0, nq, lane, E);
+      uint32_t cnt[WT], part[WT];
+      group_exposure<kIntMode, WT>(a, e0, e1, rlim, q0, nq, lane, ss, E, cnt, part);
+      const float *tab = (kIntMode && a.p_tab) ? a.p_tab + static_cast<size_t>(net) * (a.tab_max + 1)
+                                               : nullptr;
+      uint32_t pw[4] = {0u, 0u, 0u, 0u};
+      int pw_grp = -1;
 #pragma unroll
-      for (int q = 0; q < WT; ++q)
-        if (q < nq && ss[q] != 0ull) x[q] = draw_word<kInjected>(a, nd, net, i, q0 + q, lane, ss[q], E[q]);
+      for (int q = 0; q < WT; ++q) {
+        if (q < nq && ss[q] != 0ull) {
+          const float P_tab = (tab && part[q] == 0u && cnt[q] <= static_cast<uint32_t>(a.tab_max))
+                                  ? tab[cnt[q]]
+                                  : -1.0f;
+          x[q] = draw_word<kInjected>(a, nd, net, i, q0 + q, lane, ss[q], E[q], P_tab, pw, pw_grp);
+        }
+      }
     }
     // lane q stores word q of the group
     uint64_t mine = 0ull;
@@ -425,6 +480,17 @@ __global__ __launch_bounds__(kBlock) void init_kernel(InitArgs a) {
     a.S[w] = nd.m_s == 0 ? valid : (valid & ~xi);
     a.I[w] = nd.m_on == 0 ? xi : 0ull;
   }
+}
+
+// P of an exposure of c whole windows, per network: the value the draw
+// would compute from E = c * window, 1 - exp(log(1-p) * (E / 3600)).
+__global__ void p_table_kernel(float *tab, const NetDev *nets, int32_t n_net, int32_t tab_max,
+                               int32_t window) {
+  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= n_net * (tab_max + 1)) return;
+  const int net = g / (tab_max + 1), c = g - net * (tab_max + 1);
+  const float E = static_cast<float>(c * window);
+  tab[g] = 1.0f - rp::rp_expf(nets[net].log_base * (E / 3600.0f));
 }
 
 // P[k] for the last completed iteration.
@@ -581,6 +647,8 @@ struct replay_ctx {
   NetDev *d_nets = nullptr;
   int32_t *d_net_of = nullptr;
   uint32_t *d_rank_limit = nullptr;  // [n_windows][n_net] when a cap bites
+  float *d_p_tab = nullptr;          // [n_net][tab_max+1]
+  int32_t tab_max = 0;
   float *d_uniforms = nullptr;
   int cur = 0;
   int32_t steps_done = 0;
@@ -626,15 +694,15 @@ int upload(replay_ctx *ctx, T **ptr, const T *src, size_t count) {
 
 void release(replay_ctx *ctx) {
   if (!ctx) return;
-  if (ctx->stream) hipStreamSynchronize(ctx->stream);
-  if (ctx->graph) hipGraphExecDestroy(ctx->graph);
+  if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+  if (ctx->graph) (void)hipGraphExecDestroy(ctx->graph);
   ctx->graph = nullptr;
   void *bufs[] = {ctx->d_entries, ctx->d_rowptr, ctx->d_S,      ctx->d_I[0],        ctx->d_I[1],
                   ctx->d_X,       ctx->d_Xinit,  ctx->d_entry,  ctx->d_inf,         ctx->d_P,
                   ctx->d_init_count, ctx->d_counts, ctx->d_class_init, ctx->d_segs, ctx->d_nets,
-                  ctx->d_net_of,  ctx->d_rank_limit, ctx->d_uniforms};
-  for (void *b : bufs) hipFree(b);
-  if (ctx->stream) hipStreamDestroy(ctx->stream);
+                  ctx->d_net_of,  ctx->d_rank_limit, ctx->d_uniforms, ctx->d_p_tab};
+  for (void *b : bufs) (void)hipFree(b);
+  if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
   ctx->stream = nullptr;
 }
 
@@ -720,6 +788,8 @@ StepArgs step_args(replay_ctx *ctx, int32_t k) {
   a.nets = ctx->d_nets;
   a.net_of = ctx->d_net_of;
   a.rank_limit = ctx->any_cut ? ctx->d_rank_limit + static_cast<size_t>(win) * ctx->n_net : nullptr;
+  a.p_tab = ctx->d_p_tab;
+  a.tab_max = ctx->tab_max;
   a.seed = ctx->p.seed;
   a.NW = static_cast<int64_t>(ctx->N) * ctx->W;
   a.N = ctx->N;
@@ -738,7 +808,8 @@ StepArgs step_args(replay_ctx *ctx, int32_t k) {
 
 template <bool kInt, bool kInj>
 void launch_expo(const StepArgs &a, int variant, hipStream_t st) {
-  const dim3 grid((a.N + kWavesPerBlock - 1) / kWavesPerBlock);
+  const unsigned nb = (a.N + kWavesPerBlock - 1) / kWavesPerBlock;
+  const dim3 grid((nb + kXcds - 1) / kXcds * kXcds);
   if (variant == REPLAY_V_NODE16)
     hipLaunchKernelGGL((expo_kernel<kInt, kInj, 16>), grid, dim3(kBlock), 0, st, a);
   else if (variant == REPLAY_V_NODE4)
@@ -1039,6 +1110,16 @@ int finish_open(replay_ctx *ctx) {
       (rc = dev_alloc(ctx, &ctx->d_init_count, G)) ||
       (rc = dev_alloc(ctx, &ctx->d_counts, (T + 1) * G * 4)))
     return rc;
+  // probabilities of whole-window exposures, computed by the device's own exp
+  if (p.static_network_duration < (int64_t(1) << 24)) {
+    const int32_t D = static_cast<int32_t>(p.static_network_duration);
+    ctx->tab_max = std::min<int32_t>(1023, ((1 << 24) - 1) / D);
+    const int32_t n = ctx->n_net * (ctx->tab_max + 1);
+    if ((rc = dev_alloc(ctx, &ctx->d_p_tab, static_cast<size_t>(n)))) return rc;
+    hipLaunchKernelGGL(p_table_kernel, dim3((n + 255) / 256), dim3(256), 0, ctx->stream,
+                       ctx->d_p_tab, ctx->d_nets, ctx->n_net, ctx->tab_max, D);
+    HIP_OR_FAIL(ctx, hipGetLastError());
+  }
   ctx->variant = ctx->W <= 4 ? REPLAY_V_NODE4 : REPLAY_V_NODE16;
   rc = do_reset(ctx);
   if (rc != REPLAY_OK) return rc;
@@ -1090,12 +1171,12 @@ int build_graph(replay_ctx *ctx) {
   ctx->steps_done = saved_done;
   ctx->counts_done = saved_counts;
   if (rc != REPLAY_OK) {
-    if (g) hipGraphDestroy(g);
+    if (g) (void)hipGraphDestroy(g);
     return rc;
   }
   if (e != hipSuccess) return fail(ctx, REPLAY_E_DEVICE, std::string("capture: ") + hipGetErrorString(e));
   e = hipGraphInstantiate(&ctx->graph, g, nullptr, nullptr, 0);
-  hipGraphDestroy(g);
+  (void)hipGraphDestroy(g);
   if (e != hipSuccess) {
     ctx->graph = nullptr;
     return fail(ctx, REPLAY_E_DEVICE, std::string("instantiate: ") + hipGetErrorString(e));

@@ -54,11 +54,22 @@ def test_philox_matches_torch_engine(oracle, tmp_path):
 
 
 def test_uniform_definition(oracle):
-    p = oracle.make_params(M=3, seed=0x1234567890)
-    for node, rep, slot in [(0, 0, 0), (5, 2, 17), (123456, 1, 4032)]:
-        w = oracle.lib().or_philox_word(p.seed, node, rep, slot, 0)
-        assert oracle.uniform(p, slot, node, rep, 10) == (w >> 8) * 2.0 ** -24
-        assert 0.0 <= oracle.uniform(p, slot, node, rep, 10) < 1.0
+    """Replica r takes word (r >> 6) & 3 of Philox(counter {node, (r >> 8) * 64
+    + (r & 63), slot, stream}): four replicas 64 apart share one block."""
+    seed = 0x1234567890
+    for stream in (0, 9):
+        p = oracle.make_params(M=4096, seed=seed, stream=stream)
+        for node, rep, slot in [(0, 0, 0), (5, 2, 17), (123456, 1, 4032), (7, 200, 3),
+                                (7, 4095, 3), (11, 1000, 8)]:
+            ctr = (node, ((rep >> 8) << 6) | (rep & 63), slot, stream)
+            words = oracle.philox4x32_10(ctr, (seed & 0xffffffff, seed >> 32))
+            u = oracle.uniform(p, slot, node, rep, 10)
+            assert u == (words[(rep >> 6) & 3] >> 8) * 2.0 ** -24
+            assert 0.0 <= u < 1.0
+        # the four replicas sharing a block take its four words, in order
+        us = [oracle.uniform(p, 5, 3, 64 * k + 9, 10) for k in range(4)]
+        words = oracle.philox4x32_10((3, 9, 5, stream), (seed & 0xffffffff, seed >> 32))
+        assert us == [(x >> 8) * 2.0 ** -24 for x in words]
 
 
 def _ulp_dist(a: np.ndarray, b: np.ndarray) -> np.ndarray:
