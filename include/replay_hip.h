@@ -112,6 +112,8 @@ typedef struct replay_info {
 #define REPLAY_V_WORD 0       /* one pass over a node's contacts per 64-replica word */
 #define REPLAY_V_NODE4 4      /* one pass per 4 words, accumulated in registers */
 #define REPLAY_V_NODE16 16    /* one pass per 16 words, accumulated in registers */
+#define REPLAY_V_SLICED 1     /* one lane per (node, word): bit-sliced contact counts,
+                                 whole-window slices only (others use NODE4/NODE16) */
 
 typedef struct replay_ctx replay_ctx;
 

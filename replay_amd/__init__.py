@@ -68,7 +68,7 @@ class _Info(C.Structure):
 
 F_GRAPH = 1
 F_AUTOTUNE = 2
-V_WORD, V_NODE4, V_NODE16 = 0, 4, 16
+V_WORD, V_NODE4, V_NODE16, V_SLICED = 0, 4, 16, 1
 
 
 EXPORTED_SYMBOLS = (

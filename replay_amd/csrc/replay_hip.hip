@@ -159,23 +159,55 @@ __device__ __forceinline__ float row_exposure(int e0, int e1, const int4 *entrie
   }
 }
 
+// Word of lane (lane ^ w) within a 16-lane row, by DPP lane swizzles in
+// the vector pipe: quad permutations for w = 1, 2; for w = 4, the quad
+// reversal (lane ^ 3) followed by the half-row mirror (lane ^ 7); for w = 8
+// the row rotation by 8.
+template <int W>
+__device__ __forceinline__ uint32_t row_xor(uint32_t v) {
+  const int x = static_cast<int>(v);
+  if (W == 1) return static_cast<uint32_t>(__builtin_amdgcn_mov_dpp(x, 0xB1, 0xF, 0xF, true));
+  if (W == 2) return static_cast<uint32_t>(__builtin_amdgcn_mov_dpp(x, 0x4E, 0xF, 0xF, true));
+  if (W == 4)
+    return static_cast<uint32_t>(__builtin_amdgcn_mov_dpp(
+        __builtin_amdgcn_mov_dpp(x, 0x1B, 0xF, 0xF, true), 0x141, 0xF, 0xF, true));
+  return static_cast<uint32_t>(__builtin_amdgcn_mov_dpp(x, 0x128, 0xF, 0xF, true));
+}
+
+template <int K>
+__device__ __forceinline__ void count_stage(uint64_t &x, int lane) {
+  constexpr uint64_t kLow[5] = {0x5555555555555555ull, 0x3333333333333333ull,
+                                0x0F0F0F0F0F0F0F0Full, 0x00FF00FF00FF00FFull,
+                                0x0000FFFF0000FFFFull};
+  constexpr int w = 1 << K;
+  uint64_t y;
+  if (K < 4) {
+    const uint32_t lo = row_xor<w>(static_cast<uint32_t>(x));
+    const uint32_t hi = row_xor<w>(static_cast<uint32_t>(x >> 32));
+    y = (static_cast<uint64_t>(hi) << 32) | lo;
+  } else {
+    y = __shfl_xor(x, w);
+  }
+  const int sh = (lane & w) ? w : 0;
+  x = ((x >> sh) & kLow[K]) + ((y >> sh) & kLow[K]);
+}
+
 // Column counts of a 64 x 64 bit matrix held one row per lane: on exit
 // lane r holds the number of rows with bit r set.  A butterfly over the lane
 // bits: at stage k, lanes differing in bit k exchange their words, each keeps
 // the fields (of width 2^k) whose column has its own bit k and adds the
-// partner's, so field widths double while the fields per lane halve.
+// partner's, so field widths double while the fields per lane halve.  In the
+// last stage each lane sends only the 32-bit field its partner keeps.
 __device__ __forceinline__ uint32_t column_count(uint64_t x, int lane) {
-  constexpr uint64_t kLow[6] = {0x5555555555555555ull, 0x3333333333333333ull,
-                                0x0F0F0F0F0F0F0F0Full, 0x00FF00FF00FF00FFull,
-                                0x0000FFFF0000FFFFull, 0x00000000FFFFFFFFull};
-#pragma unroll
-  for (int k = 0; k < 6; ++k) {
-    const int w = 1 << k;
-    const uint64_t y = __shfl_xor(x, w);
-    const int sh = (lane & w) ? w : 0;
-    x = ((x >> sh) & kLow[k]) + ((y >> sh) & kLow[k]);
-  }
-  return static_cast<uint32_t>(x);
+  count_stage<0>(x, lane);
+  count_stage<1>(x, lane);
+  count_stage<2>(x, lane);
+  count_stage<3>(x, lane);
+  count_stage<4>(x, lane);
+  const bool upper = lane & 32;
+  const uint32_t keep = upper ? static_cast<uint32_t>(x >> 32) : static_cast<uint32_t>(x);
+  const uint32_t give = upper ? static_cast<uint32_t>(x) : static_cast<uint32_t>(x >> 32);
+  return keep + static_cast<uint32_t>(__shfl_xor(static_cast<int>(give), 32));
 }
 
 // Exposure of the words q0 .. q0+nq-1 (nq <= WT) of node i for one window.
@@ -402,6 +434,261 @@ __global__ __launch_bounds__(kBlock) void expo_kernel(StepArgs a) {
   }
 }
 
+// Carry-save adder of three 64-replica words (Harley and Seal): per replica
+// bit, a + b + c = 2 * hi + lo.
+__device__ __forceinline__ void csa(uint64_t &hi, uint64_t &lo, uint64_t a, uint64_t b,
+                                    uint64_t c) {
+  const uint64_t u = a ^ b;
+  hi = (u & c) | (~u & a);  // majority of a, b, c
+  lo = u ^ c;
+}
+
+constexpr int kPartialWords = 256;  // partial-overlap words recorded per wavefront
+constexpr int kTabLds = 1024;       // probability-table entries staged in LDS per block
+
+// Phase 1, bit-sliced form (integer windows only).  A wavefront covers
+// G = 64 / W nodes with one lane per (node, 64-replica word).  Each lane walks
+// its node's contact entries eight at a time, gathers the neighbour's word for
+// every whole-window contact and adds the eight words into bit-sliced
+// counters (plane b holds bit b of every replica's count) with a carry-save
+// adder tree: about one 32-bit operation per replica word and contact, and no
+// cross-lane traffic.  Contacts that overlap only part of the window keep
+// their overlap and gathered words in LDS (every lane of a node walks the same
+// entries, so all of them agree on the slot).  Then the wavefront takes its
+// nodes one at a time with one lane per replica: it reads each replica's count
+// out of the owning lane's planes, adds the recorded partial overlaps, and
+// draws as expo_kernel does.  The exposure is the same exact integer,
+// L * count + sum of partial overlaps.
+template <int NB, bool kInjected>
+__global__ __launch_bounds__(kBlock) void sliced_kernel(StepArgs a, int G) {
+  extern __shared__ uint64_t lds[];
+  const int lane = threadIdx.x & (kWave - 1);
+  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
+  const int W = a.W;
+  const int K = kPartialWords / (G * W);  // partial slots per node, >= 4
+  uint64_t *part_words = lds + static_cast<size_t>(wv) * kPartialWords;  // [node slot][K][W]
+  int32_t *ints = reinterpret_cast<int32_t *>(lds + kWavesPerBlock * kPartialWords);
+  float *tab_lds = reinterpret_cast<float *>(ints);  // [kTabLds]
+  int32_t *n_part = ints + kTabLds + wv * (kWave + kPartialWords);  // [node slot]
+  int32_t *part_ov = n_part + kWave;                                // [node slot][K]
+  const unsigned per_xcd = gridDim.x / kXcds;
+  const unsigned lb = (blockIdx.x % kXcds) * per_xcd + blockIdx.x / kXcds;
+  const int block_base = static_cast<int>(lb) * kWavesPerBlock * G;
+  const int base = block_base + wv * G;
+  const int L = a.we_rel - a.ws_rel;
+
+  // the block's first network's probability table, staged in LDS
+  const int tab_net = block_base < a.N ? a.net_of[block_base] : 0;
+  const int n_tab = a.p_tab ? min(a.tab_max + 1, kTabLds) : 0;
+  for (int t = threadIdx.x; t < n_tab; t += kBlock)
+    tab_lds[t] = a.p_tab[static_cast<size_t>(tab_net) * (a.tab_max + 1) + t];
+
+  // ---- counting: lane = (node slot g, word q)
+  const int g_l = lane / W;
+  const int q_l = lane - g_l * W;
+  const int i_l = base + g_l;
+  int e0_l = 0, e1_l = 0, net_l = 0, off_l = 0;
+  uint32_t rlim_l = UINT32_MAX, stream_l = 0;
+  float logb_l = 0.0f;
+  uint64_t ss_l = 0ull;
+  if (g_l < G && i_l < a.N) {
+    e0_l = a.rowptr[i_l];
+    e1_l = a.rowptr[i_l + 1];
+    net_l = a.net_of[i_l];
+    if (a.rank_limit) rlim_l = a.rank_limit[net_l];
+    ss_l = a.S[static_cast<size_t>(i_l) * W + q_l];
+    const NetDev &nd = a.nets[net_l];
+    logb_l = nd.log_base;
+    off_l = nd.node_off;
+    stream_l = nd.stream;
+  }
+  n_part[lane] = 0;
+  __builtin_amdgcn_wave_barrier();
+  uint64_t p[NB];
+#pragma unroll
+  for (int b = 0; b < NB; ++b) p[b] = 0ull;
+  {
+    int np = 0;
+    const uint64_t *col_words = a.I_cur + q_l;
+    // words whose replicas are all non-susceptible need no counts
+    const int e_end = (ss_l != 0ull || kInjected) ? e1_l : e0_l;
+    int4 en[8];
+    if (e0_l < e_end) {
+#pragma unroll
+      for (int t = 0; t < 8; ++t) en[t] = a.entries[min(e0_l + t, e_end - 1)];
+    }
+    for (int e = e0_l; e < e_end; e += 8) {
+      uint64_t m[8];
+      int ov[8];
+      bool ok[8], part_any = false;
+#pragma unroll
+      for (int t = 0; t < 8; ++t) {
+        ov[t] = min(en[t].z, a.we_rel) - max(en[t].y, a.ws_rel);
+        ok[t] = (e + t < e_end) & (ov[t] > 0) & (static_cast<uint32_t>(en[t].w) <= rlim_l);
+        part_any |= ok[t] & (ov[t] != L);
+        // N < 2^24 and N * W < 2^32 (checked on the host): 24-bit multiply
+        m[t] = col_words[__umul24(static_cast<uint32_t>(ok[t] ? en[t].x : 0), static_cast<uint32_t>(W))];
+      }
+      // the next eight entries load while these words are in flight
+      if (e + 8 < e_end) {
+#pragma unroll
+        for (int t = 0; t < 8; ++t) en[t] = a.entries[min(e + 8 + t, e_end - 1)];
+      }
+      uint64_t w[8];
+#pragma unroll
+      for (int t = 0; t < 8; ++t) w[t] = (ok[t] & (ov[t] == L)) ? m[t] : 0ull;
+      if (__ballot(part_any) != 0ull) {  // partial overlaps: rare
+#pragma unroll
+        for (int t = 0; t < 8; ++t) {
+          if (ok[t] & (ov[t] != L)) {
+            if (np < K) {
+              part_words[(g_l * K + np) * W + q_l] = m[t];
+              part_ov[g_l * K + np] = ov[t];  // the same value from every lane of the node
+            }
+            ++np;
+          }
+        }
+      }
+      uint64_t twosA, twosB, foursA, foursB, eights;
+      csa(twosA, p[0], p[0], w[0], w[1]);
+      csa(twosB, p[0], p[0], w[2], w[3]);
+      csa(foursA, p[1], p[1], twosA, twosB);
+      csa(twosA, p[0], p[0], w[4], w[5]);
+      csa(twosB, p[0], p[0], w[6], w[7]);
+      csa(foursB, p[1], p[1], twosA, twosB);
+      csa(eights, p[2], p[2], foursA, foursB);
+#pragma unroll
+      for (int b = 3; b < NB; ++b) {  // counts stay below 2^NB: no carry out
+        const uint64_t c = p[b] & eights;
+        p[b] ^= eights;
+        eights = c;
+      }
+    }
+    // lanes of a word without susceptible replicas skip the walk; the others
+    // all found the same np
+    if (np > 0) n_part[g_l] = np;
+  }
+  int hb_l = -1;
+#pragma unroll
+  for (int b = 0; b < NB; ++b)
+    if (p[b] != 0ull) hb_l = b;
+  __syncthreads();
+
+  // ---- draws: lane = replica within a word, one node at a time
+  const bool hi_half = lane >= 32;
+  const int bitpos = lane & 31;
+  for (int g = 0; g < G; ++g) {
+    const int i = base + g;
+    if (i >= a.N) break;
+    const int slot0 = g * W;
+    const int net = __builtin_amdgcn_readlane(net_l, slot0);
+    const float log_base = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(logb_l), slot0));
+    const int node_off = __builtin_amdgcn_readlane(off_l, slot0);
+    const uint32_t stream = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(stream_l), slot0));
+    uint64_t *X_k = a.X + static_cast<size_t>(a.k % a.depth) * a.NW + static_cast<size_t>(i) * W;
+    const int np = __builtin_amdgcn_readfirstlane(n_part[g]);
+    const bool tab_here = net == tab_net;
+    const float *tab = a.p_tab ? a.p_tab + static_cast<size_t>(net) * (a.tab_max + 1) : nullptr;
+    uint64_t mine = 0ull;
+    for (int q0 = 0; q0 < W; q0 += 4) {
+      const int nq = min(4, W - q0);
+      float E[4];
+      uint32_t cnt[4], part[4];
+      uint64_t ss[4];
+      bool need_any = false;
+#pragma unroll
+      for (int t = 0; t < 4; ++t) {
+        ss[t] = 0ull;
+        E[t] = 0.0f;
+        cnt[t] = part[t] = 0u;
+        if (t >= nq) continue;
+        const int q = q0 + t;
+        const int slot = slot0 + q;
+        ss[t] = (static_cast<uint64_t>(static_cast<uint32_t>(
+                     __builtin_amdgcn_readlane(static_cast<int>(ss_l >> 32), slot)))
+                 << 32) |
+                static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(ss_l), slot));
+        if (ss[t] == 0ull) continue;
+        const int hb = __builtin_amdgcn_readlane(hb_l, slot);
+        uint32_t c = 0;
+#pragma unroll
+        for (int b = 0; b < NB; ++b) {
+          if (b <= hb) {
+            const uint32_t lo = static_cast<uint32_t>(
+                __builtin_amdgcn_readlane(static_cast<int>(static_cast<uint32_t>(p[b])), slot));
+            const uint32_t hi = static_cast<uint32_t>(
+                __builtin_amdgcn_readlane(static_cast<int>(static_cast<uint32_t>(p[b] >> 32)), slot));
+            c |= (((hi_half ? hi : lo) >> bitpos) & 1u) << b;
+          }
+        }
+        uint32_t add = 0;
+        if (np > 0 && np <= K) {
+          for (int j = 0; j < np; ++j) {
+            const uint64_t mw = part_words[(g * K + j) * W + q];
+            add += static_cast<uint32_t>((mw >> lane) & 1ull) *
+                   static_cast<uint32_t>(part_ov[g * K + j]);
+          }
+        } else if (np > K) {  // slots overflowed: scan the node's entries again
+          const int e0 = __builtin_amdgcn_readlane(e0_l, slot0);
+          const int e1 = __builtin_amdgcn_readlane(e1_l, slot0);
+          const uint32_t rlim = static_cast<uint32_t>(
+              __builtin_amdgcn_readlane(static_cast<int>(rlim_l), slot0));
+          const uint64_t *cw = a.I_cur + q;
+          for (int ee = e0; ee < e1; ++ee) {
+            const int4 en = a.entries[ee];
+            const int ov = min(en.z, a.we_rel) - max(en.y, a.ws_rel);
+            if (!(ov > 0 && ov != L && static_cast<uint32_t>(en.w) <= rlim)) continue;
+            const uint64_t mw = cw[static_cast<size_t>(en.x) * W];
+            add += static_cast<uint32_t>((mw >> lane) & 1ull) * static_cast<uint32_t>(ov);
+          }
+        }
+        cnt[t] = c;
+        part[t] = add;
+        E[t] = static_cast<float>(static_cast<uint32_t>(L) * c + add);
+        need_any |= ((ss[t] >> lane) & 1ull) && (kInjected || E[t] > 0.0f);
+      }
+      if (__ballot(need_any) == 0ull) continue;  // no draws in these words
+      uint32_t pw[4] = {0u, 0u, 0u, 0u};
+      if (!kInjected)
+        rp::philox4(a.seed, static_cast<uint32_t>(i - node_off),
+                    (static_cast<uint32_t>(q0 >> 2) << 6) | static_cast<uint32_t>(lane), a.slot,
+                    stream, pw);
+#pragma unroll
+      for (int t = 0; t < 4; ++t) {
+        if (t >= nq) continue;
+        const int q = q0 + t;
+        const int r = q * kWave + lane;
+        bool infect = false;
+        if (((ss[t] >> lane) & 1ull) && (kInjected || E[t] > 0.0f)) {
+          float P;
+          if (part[t] == 0u && tab_here && cnt[t] < static_cast<uint32_t>(n_tab))
+            P = tab_lds[cnt[t]];
+          else if (tab && part[t] == 0u && cnt[t] <= static_cast<uint32_t>(a.tab_max))
+            P = tab[cnt[t]];
+          else
+            P = 1.0f - rp::rp_expf(log_base * (E[t] / 3600.0f));
+          const float u = kInjected ? a.uniforms[static_cast<size_t>(r) * a.N + i] : rp::u24(pw[t]);
+          infect = P > u;
+        }
+        if (infect) {
+          a.entry[static_cast<size_t>(i) * a.Mp + r] = 2 * a.k + 1;
+          atomicAdd(a.inf_count + (static_cast<size_t>(a.k) * a.n_net + net) * a.M + r, 1);
+        }
+        const uint64_t x = __ballot(infect);
+        if (lane == q) mine = x;
+      }
+    }
+    if (lane < W) X_k[lane] = mine;
+  }
+}
+
+constexpr size_t sliced_lds_bytes(int nb) {
+  (void)nb;
+  return static_cast<size_t>(kWavesPerBlock) * kPartialWords * sizeof(uint64_t) +
+         (kTabLds + static_cast<size_t>(kWavesPerBlock) * (kWave + kPartialWords)) *
+             sizeof(int32_t);
+}
+
 // Phase 2 of an iteration: one thread per word applies the timer flips for
 // iteration k+1 and advances the running prefix of the infection counts,
 // P[k] = P[k-1] + inf_count[k-1].
@@ -622,6 +909,7 @@ struct replay_ctx {
   std::vector<int32_t> win_epoch;
   std::vector<uint8_t> win_int;
   std::vector<int64_t> epoch_t0, epoch_entry0;
+  std::vector<int32_t> epoch_maxrow;  // longest row of each epoch
   int64_t n_entries = 0;
   int32_t windows_per_epoch = 1;
   bool any_cut = false;
@@ -818,9 +1106,43 @@ void launch_expo(const StepArgs &a, int variant, hipStream_t st) {
     hipLaunchKernelGGL((expo_kernel<kInt, kInj, 1>), grid, dim3(kBlock), 0, st, a);
 }
 
-// One iteration: exposure + draws per node, then the word flips.
+// Planes the bit-sliced counters need for rows of up to maxrow contacts
+// (0: rows too long for the bit-sliced form).
+int sliced_planes(int32_t maxrow) {
+  if (maxrow < (1 << 8)) return 8;
+  if (maxrow < (1 << 12)) return 12;
+  if (maxrow < (1 << 16)) return 16;
+  return 0;
+}
+
+template <int NB>
+void launch_sliced_nb(const StepArgs &a, bool injected, hipStream_t st) {
+  const int G = kWave / a.W;
+  const int nodes_per_block = kWavesPerBlock * G;
+  const unsigned nb = (a.N + nodes_per_block - 1) / nodes_per_block;
+  const dim3 grid((nb + kXcds - 1) / kXcds * kXcds);
+  if (injected)
+    hipLaunchKernelGGL((sliced_kernel<NB, true>), grid, dim3(kBlock), sliced_lds_bytes(NB), st, a, G);
+  else
+    hipLaunchKernelGGL((sliced_kernel<NB, false>), grid, dim3(kBlock), sliced_lds_bytes(NB), st, a,
+                       G);
+}
+
+// One iteration: exposure + draws per node, then the word flips.  nb_planes
+// > 0 selects the bit-sliced exposure kernel (integer windows only).
 void launch_step(const StepArgs &a, int variant, bool int_mode, bool injected, dim3 grid,
-                 hipStream_t st) {
+                 hipStream_t st, int nb_planes = 0) {
+  if (variant == REPLAY_V_SLICED && int_mode && nb_planes > 0) {
+    if (nb_planes == 8)
+      launch_sliced_nb<8>(a, injected, st);
+    else if (nb_planes == 12)
+      launch_sliced_nb<12>(a, injected, st);
+    else
+      launch_sliced_nb<16>(a, injected, st);
+    hipLaunchKernelGGL(update_kernel, grid, dim3(kBlock), 0, st, a);
+    return;
+  }
+  if (variant == REPLAY_V_SLICED) variant = a.W <= 4 ? REPLAY_V_NODE4 : REPLAY_V_NODE16;
   if (int_mode && !injected)
     launch_expo<true, false>(a, variant, st);
   else if (int_mode)
@@ -840,7 +1162,9 @@ dim3 step_grid(const replay_ctx *ctx) {
 }
 
 bool variant_applies(const replay_ctx *ctx, int v) {
-  (void)ctx;
+  if (v == REPLAY_V_SLICED)
+    return ctx->W <= kWave && ctx->N < (1 << 24) &&
+           static_cast<int64_t>(ctx->N) * ctx->W < (int64_t(1) << 32);
   return v == REPLAY_V_WORD || v == REPLAY_V_NODE4 || v == REPLAY_V_NODE16;
 }
 
@@ -849,8 +1173,10 @@ int enqueue_steps(replay_ctx *ctx, int32_t steps) {
   for (int32_t s = 0; s < steps; ++s) {
     const int32_t k = ctx->steps_done;
     StepArgs a = step_args(ctx, k);
-    const bool int_mode = ctx->win_int[ctx->iter_window[k]] != 0;
-    launch_step(a, ctx->variant, int_mode, a.uniforms != nullptr, grid, ctx->stream);
+    const int32_t win = ctx->iter_window[k];
+    const bool int_mode = ctx->win_int[win] != 0;
+    launch_step(a, ctx->variant, int_mode, a.uniforms != nullptr, grid, ctx->stream,
+                sliced_planes(ctx->epoch_maxrow[ctx->win_epoch[win]]));
     HIP_OR_FAIL(ctx, hipGetLastError());
     ctx->cur ^= 1;
     ctx->steps_done = k + 1;
@@ -1037,6 +1363,7 @@ int finish_open(replay_ctx *ctx) {
   ctx->any_cut = ctx->truncated > 0;
   std::vector<int32_t> rowptr(static_cast<size_t>(n_ep) * (ctx->N + 1));
   ctx->epoch_entry0.assign(static_cast<size_t>(n_ep) + 1, 0);
+  ctx->epoch_maxrow.assign(static_cast<size_t>(n_ep), 0);
   int64_t total = 0;
   for (int32_t e = 0; e < n_ep; ++e) {
     ctx->epoch_entry0[e] = total;
@@ -1052,6 +1379,10 @@ int finish_open(replay_ctx *ctx) {
     }
     rowptr[static_cast<size_t>(e) * (ctx->N + 1) + ctx->N] = static_cast<int32_t>(base);
     total += base;
+    int32_t mr = 0;
+    const int32_t *rpe = &rowptr[static_cast<size_t>(e) * (ctx->N + 1)];
+    for (int32_t r = 0; r < ctx->N; ++r) mr = std::max(mr, rpe[r + 1] - rpe[r]);
+    ctx->epoch_maxrow[e] = mr;
   }
   ctx->epoch_entry0[n_ep] = total;
   ctx->n_entries = total;
@@ -1376,7 +1707,7 @@ int32_t replay_autotune(replay_ctx *ctx, int32_t steps, int32_t *variant) {
   if (!ctx || !variant) return REPLAY_E_ARG;
   steps = std::min(steps, ctx->T);
   HIP_OR_FAIL(ctx, hipSetDevice(ctx->p.device));
-  const int candidates[] = {REPLAY_V_WORD, REPLAY_V_NODE4, REPLAY_V_NODE16};
+  const int candidates[] = {REPLAY_V_WORD, REPLAY_V_NODE4, REPLAY_V_NODE16, REPLAY_V_SLICED};
   int best = ctx->variant;
   double best_ms = 1e30;
   for (int v : candidates) {

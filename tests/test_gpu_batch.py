@@ -90,13 +90,14 @@ def test_variants_agree(M):
     shared = Params(M=M, step_size=600.0, static_network_duration=600, iterations=60, seed=2)
     results = []
     with Simulation.from_networks(nets, shared) as sim:
-        for v in (replay_amd.V_WORD, replay_amd.V_NODE4, replay_amd.V_NODE16):
+        for v in (replay_amd.V_WORD, replay_amd.V_NODE4, replay_amd.V_NODE16, replay_amd.V_SLICED):
             sim.set_variant(v)
             sim.reset()
             sim.run()
             results.append((sim.counts(), sim.state()))
         best = sim.autotune(20)
-        assert best in (replay_amd.V_WORD, replay_amd.V_NODE4, replay_amd.V_NODE16)
+        assert best in (replay_amd.V_WORD, replay_amd.V_NODE4, replay_amd.V_NODE16,
+                        replay_amd.V_SLICED)
         sim.run()
         results.append((sim.counts(), sim.state()))
     for c, s in results[1:]:
