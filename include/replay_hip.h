@@ -222,6 +222,38 @@ int32_t replay_get_slice(replay_ctx *ctx, int32_t k, int32_t *rowptr,
 int32_t replay_get_exposure(replay_ctx *ctx, int32_t k, float *out,
                             int64_t n_out);
 
+/* ---- Differentiable marginal-probability form (no counterpart in the
+   reference, which only samples).  Individual-level SEIR probabilities per
+   node and parameter set over the same duration-weighted slices as the
+   stochastic step: lam = 1 - exp(-beta * phi_i * E_i / 3600) with
+   E_i = sum_j A_k[i,j] iota_j, exponential exposed and infectious sojourns
+   (rates sigma, gamma per hour), initial iota = initial_infected.  Forward
+   curves and a hand-written adjoint (transposed-slice product plus the
+   transition adjoints), with checkpoints every checkpoint_every iterations
+   recomputed during the backward pass.  All data pointers of the forward and
+   backward calls are DEVICE pointers (fp32); stream is a hipStream_t (NULL:
+   the context's stream).  Results are deterministic. */
+typedef struct replay_marginal replay_marginal;
+
+/* n_sets parameter sets over the context's network(s) and iterations. */
+int32_t replay_marginal_create(replay_ctx *ctx, int32_t n_sets,
+                               int32_t checkpoint_every, replay_marginal **out);
+
+/* beta, sigma, gamma: [n_sets]; phi: [N] per-node susceptibility; curves out:
+   [(iterations + 1)][n_sets][4] expected (S, E, I, R) counts. */
+int32_t replay_marginal_forward(replay_marginal *m, const float *beta,
+                                const float *sigma, const float *gamma,
+                                const float *phi, float *curves, void *stream);
+
+/* Gradients of a loss with dL/dcurves = grad_curves (same shape as curves)
+   with respect to the parameters of the last forward call. */
+int32_t replay_marginal_backward(replay_marginal *m, const float *grad_curves,
+                                 float *grad_beta, float *grad_sigma,
+                                 float *grad_gamma, float *grad_phi,
+                                 void *stream);
+
+void replay_marginal_destroy(replay_marginal *m);
+
 /* Message of the last failed call on ctx (or of the last failed open on
    this thread when ctx is NULL). */
 const char *replay_last_error(replay_ctx *ctx);

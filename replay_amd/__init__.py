@@ -78,6 +78,8 @@ EXPORTED_SYMBOLS = (
     "replay_get_state", "replay_iteration_time", "replay_set_uniforms", "replay_get_slice",
     "replay_get_exposure", "replay_last_error", "replay_error_string", "replay_close",
     "replay_open_batch", "replay_get_network_offsets", "replay_autotune", "replay_set_variant",
+    "replay_marginal_create", "replay_marginal_forward", "replay_marginal_backward",
+    "replay_marginal_destroy",
 )
 
 _lib = None
@@ -127,6 +129,14 @@ def load_library() -> C.CDLL:
     L.replay_error_string.restype = C.c_char_p
     L.replay_close.argtypes = [ctx_p]
     L.replay_close.restype = None
+    vp = C.c_void_p
+    L.replay_marginal_create.argtypes = [ctx_p, C.c_int32, C.c_int32, P(vp)]
+    L.replay_marginal_forward.argtypes = [vp, vp, vp, vp, vp, vp, vp]
+    L.replay_marginal_backward.argtypes = [vp, vp, vp, vp, vp, vp, vp]
+    L.replay_marginal_destroy.argtypes = [vp]
+    L.replay_marginal_destroy.restype = None
+    for name in ("replay_marginal_create", "replay_marginal_forward", "replay_marginal_backward"):
+        getattr(L, name).restype = C.c_int32
     for name in ("replay_open", "replay_open_contacts", "replay_get_info", "replay_reset",
                  "replay_run", "replay_run_timed", "replay_sync", "replay_get_counts", "replay_get_state",
                  "replay_set_uniforms", "replay_get_slice", "replay_get_exposure",

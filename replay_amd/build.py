@@ -53,10 +53,13 @@ def build(quiet: bool = True) -> str:
         obj = os.path.join(BUILD, src.replace(".cpp", ".o"))
         _run(["g++", *CXX_FLAGS, "-c", os.path.join(CSRC, src), "-o", obj], quiet)
         host_objs.append(obj)
-    hip_o = os.path.join(BUILD, "replay_hip.o")
-    _run([HIPCC, *HIP_FLAGS, "-c", os.path.join(CSRC, "replay_hip.hip"), "-o", hip_o], quiet)
+    hip_objs = []
+    for src in ("replay_hip.hip", "marginal.hip"):
+        obj = os.path.join(BUILD, src.replace(".hip", ".o"))
+        _run([HIPCC, *HIP_FLAGS, "-c", os.path.join(CSRC, src), "-o", obj], quiet)
+        hip_objs.append(obj)
     tmp = LIB + ".tmp"
-    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-pthread", *host_objs, hip_o,
+    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-pthread", *host_objs, *hip_objs,
           "-o", tmp], quiet)
     os.replace(tmp, LIB)
     _run(["g++", "-O2", "-std=c++17", f"-I{INCLUDE}", os.path.join(CSRC, "replay_main.cpp"),

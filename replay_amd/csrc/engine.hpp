@@ -1,0 +1,180 @@
+// engine.hpp -- definitions shared by the translation units of
+// libreplay_hip: launch constants, the device-side argument structs, the
+// host-side context and the error helpers.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <stdint.h>
+
+#include <string>
+#include <vector>
+
+#include "device_math.hpp"
+#include "ingest.hpp"
+#include "replay_hip.h"
+
+namespace rp {
+
+constexpr int kWave = 64;
+constexpr int kBlock = 256;
+constexpr int kWavesPerBlock = kBlock / kWave;
+constexpr unsigned kXcds = 8;  // MI355X accelerator complex dies
+constexpr int32_t kNever = INT32_MAX;
+constexpr int kMaxSegs = 16;
+
+struct Segs {  // runs of the tabulated class over n = 1..T for new infections
+  int32_t n;
+  int32_t a[kMaxSegs];    // first offset of the run
+  int32_t b[kMaxSegs];    // one past the last offset
+  int32_t cls[kMaxSegs];  // 1 E, 2 I, 3 R
+};
+
+// Constants of one network on the device.
+struct NetDev {
+  float log_base;    // logf(1 - infect_prob), computed on the host
+  float p0;          // initial_infected
+  int32_t node_off;  // first row
+  uint32_t stream;   // Philox counter word 3
+  int32_t n_on, n_off, n_s;  // flip offsets of a new infection (kNever: none)
+  int32_t m_on, m_off, m_s;  // the same for the initial seeds
+};
+
+struct StepArgs {
+  const int32_t *rowptr;      // slice epoch rowptr (N+1), epoch-relative entries
+  const int4 *entries;        // slice epoch entries {col, ts, te, rank}
+  const uint64_t *I_cur;      // infectious bits at iteration k
+  uint64_t *I_next;           // infectious bits at iteration k+1
+  uint64_t *S;                // susceptible bits, k -> k+1 in place
+  uint64_t *X;                // infection ring, depth x NW
+  const uint64_t *X_init;
+  int32_t *entry;             // [N][Mp]
+  int32_t *inf_count;         // [T+1][n_net][M]
+  int32_t *P;                 // prefix sums of inf_count, [T+2][n_net][M]
+  const float *uniforms;      // injected stream for this slot, [M][N], or null
+  const NetDev *nets;
+  const int32_t *net_of;      // node -> network
+  const uint32_t *rank_limit; // [n_net] of this window, or null (no cap bites)
+  const float *p_tab;         // [n_net][tab_max+1]: P of c whole windows, or null
+  int32_t tab_max;
+  uint64_t seed;
+  int64_t NW;                 // N * W words
+  int32_t N, M, W, Mp, n_net, depth;
+  int32_t k;
+  int32_t ws_rel, we_rel;
+  float scale;
+  uint32_t slot;
+};
+
+}  // namespace rp
+
+// One network of a context, on the host.
+struct NetHost {
+  rp::Network net;
+  rp::Plan plan;
+  rp::Slices slices;
+  float p0 = 0.0f, beta = 0.0f, log_base = 0.0f;
+  int32_t Ed = 0, Id = 0, Rd = 0, upper = 0, medium = 0, lower = 0;
+  uint32_t stream = 0;
+  int32_t node_off = 0;
+  // timer tables
+  std::vector<int32_t> lut_new, lut_init;  // c(n), n = 0..T
+  int32_t n_on = rp::kNever, n_off = rp::kNever, n_s = rp::kNever;
+  int32_t m_on = rp::kNever, m_off = rp::kNever, m_s = rp::kNever;
+  rp::Segs segs{};
+  std::vector<int8_t> class_init;  // class of c_init(j), j = 0..T
+};
+
+struct replay_ctx {
+  replay_params p{};
+  std::vector<NetHost> nets;
+  int32_t n_net = 0;
+  int32_t N = 0, M = 0, W = 0, Mp = 0, T = 0, depth = 1;
+  // merged plan and slices
+  std::vector<int64_t> iter_time;
+  std::vector<int32_t> iter_window;
+  std::vector<int64_t> win_ws, win_we;
+  std::vector<float> win_scale;
+  std::vector<int32_t> win_epoch;
+  std::vector<uint8_t> win_int;
+  std::vector<int64_t> epoch_t0, epoch_entry0;
+  std::vector<int32_t> epoch_maxrow;  // longest row of each epoch
+  int64_t n_entries = 0;
+  int32_t windows_per_epoch = 1;
+  bool any_cut = false;
+  int truncated = 0;
+  int64_t n_contacts = 0, nlists = 0;
+  int32_t variant = REPLAY_V_WORD;
+  hipStream_t stream = nullptr;
+  hipGraphExec_t graph = nullptr;
+  // device
+  int4 *d_entries = nullptr;
+  int32_t *d_rowptr = nullptr;
+  uint64_t *d_S = nullptr;
+  uint64_t *d_I[2] = {nullptr, nullptr};
+  uint64_t *d_X = nullptr;
+  uint64_t *d_Xinit = nullptr;
+  int32_t *d_entry = nullptr;
+  int32_t *d_inf = nullptr;
+  int32_t *d_P = nullptr;
+  int32_t *d_init_count = nullptr;
+  int32_t *d_counts = nullptr;
+  int8_t *d_class_init = nullptr;
+  rp::Segs *d_segs = nullptr;
+  rp::NetDev *d_nets = nullptr;
+  int32_t *d_net_of = nullptr;
+  uint32_t *d_rank_limit = nullptr;  // [n_windows][n_net] when a cap bites
+  float *d_p_tab = nullptr;          // [n_net][tab_max+1]
+  int32_t tab_max = 0;
+  float *d_uniforms = nullptr;
+  int cur = 0;
+  int32_t steps_done = 0;
+  int32_t counts_done = -1;
+  std::string err;
+};
+
+namespace rp {
+
+// Message of the last failed open on this thread (no context exists yet).
+extern thread_local std::string g_open_error;
+
+inline int fail(replay_ctx *ctx, int code, const std::string &msg) {
+  if (ctx)
+    ctx->err = msg;
+  else
+    g_open_error = msg;
+  return code;
+}
+
+#define HIP_OR_FAIL(ctx, call)                                                         \
+  do {                                                                                 \
+    hipError_t e_ = (call);                                                            \
+    if (e_ != hipSuccess)                                                              \
+      return fail((ctx), REPLAY_E_DEVICE, std::string(#call) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+template <typename T>
+int dev_alloc(replay_ctx *ctx, T **ptr, size_t count) {
+  if (count == 0) count = 1;
+  hipError_t e = hipMalloc(reinterpret_cast<void **>(ptr), count * sizeof(T));
+  if (e != hipSuccess)
+    return fail(ctx, REPLAY_E_NOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
+  return REPLAY_OK;
+}
+
+template <typename T>
+int upload(replay_ctx *ctx, T **ptr, const T *src, size_t count) {
+  int rc = dev_alloc(ctx, ptr, count);
+  if (rc) return rc;
+  if (count)
+    HIP_OR_FAIL(ctx, hipMemcpyAsync(*ptr, src, count * sizeof(T), hipMemcpyHostToDevice,
+                                    ctx->stream));
+  return REPLAY_OK;
+}
+
+
+// Device arguments of iteration k: the slice epoch and window, the state
+// buffers of the current parity, the draw slot.
+StepArgs step_args(replay_ctx *ctx, int32_t k);
+
+}  // namespace rp

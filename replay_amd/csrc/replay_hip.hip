@@ -25,17 +25,17 @@
 // inf_count over the offset ranges where the tabulated class
 // (cu:530-550) is E, I or R.
 //
-// Step kernel: a block owns 256 consecutive words.  Every thread loads its
-// word's S; words of nodes that have contacts in the current slice and still
-// hold susceptible replicas are queued in LDS, and the block's wavefronts
-// take the queue one item at a time with one lane per replica: each walks the
-// node's packed contact entries in ascending column order, adds each
-// contact's overlap with the window where the neighbour's infectious bit is
-// set, converts exposure to 1 - exp(log(1-p) * E/3600), draws a Philox
-// uniform and ballots the infections.  Then every thread applies its word's
-// flips.  Variant NODE4/NODE16 queues one item per node and keeps all of the
-// node's words in registers (integer exposure only); WORD queues one item
-// per word.
+// Each iteration is two kernels.  expo_kernel gives a wavefront to one node:
+// the lanes load 64 of the node's packed contact entries at a time
+// (coalesced), gather the neighbours' infectious words and reduce them to
+// per-replica exposure -- whole-window contacts by a butterfly column count
+// across the lanes, partial overlaps one contact at a time -- then convert
+// exposure to 1 - exp(log(1-p) * E/3600), draw Philox uniforms and ballot the
+// infections into the ring slot of the iteration.  update_kernel then gives a
+// thread to each word and applies the flips above.  Variants NODE4/NODE16
+// keep 4 or 16 of a node's words in registers per pass over its contacts
+// (WORD: one); SLICED gives each lane one (node, word) and counts contacts in
+// bit planes.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -48,61 +48,16 @@
 #include <thread>
 #include <vector>
 
-#include "device_math.hpp"
-#include "ingest.hpp"
-#include "replay_hip.h"
+#include "engine.hpp"
+
+namespace rp {
+thread_local std::string g_open_error;
+}  // namespace rp
+
+using namespace rp;
 
 namespace {
 
-constexpr int kWave = 64;
-constexpr int kBlock = 256;
-constexpr int kWavesPerBlock = kBlock / kWave;
-constexpr unsigned kXcds = 8;  // MI355X accelerator complex dies
-constexpr int32_t kNever = INT32_MAX;
-constexpr int kMaxSegs = 16;
-
-struct Segs {  // runs of the tabulated class over n = 1..T for new infections
-  int32_t n;
-  int32_t a[kMaxSegs];    // first offset of the run
-  int32_t b[kMaxSegs];    // one past the last offset
-  int32_t cls[kMaxSegs];  // 1 E, 2 I, 3 R
-};
-
-// Constants of one network on the device.
-struct NetDev {
-  float log_base;    // logf(1 - infect_prob), computed on the host
-  float p0;          // initial_infected
-  int32_t node_off;  // first row
-  uint32_t stream;   // Philox counter word 3
-  int32_t n_on, n_off, n_s;  // flip offsets of a new infection (kNever: none)
-  int32_t m_on, m_off, m_s;  // the same for the initial seeds
-};
-
-struct StepArgs {
-  const int32_t *rowptr;      // slice epoch rowptr (N+1), epoch-relative entries
-  const int4 *entries;        // slice epoch entries {col, ts, te, rank}
-  const uint64_t *I_cur;      // infectious bits at iteration k
-  uint64_t *I_next;           // infectious bits at iteration k+1
-  uint64_t *S;                // susceptible bits, k -> k+1 in place
-  uint64_t *X;                // infection ring, depth x NW
-  const uint64_t *X_init;
-  int32_t *entry;             // [N][Mp]
-  int32_t *inf_count;         // [T+1][n_net][M]
-  int32_t *P;                 // prefix sums of inf_count, [T+2][n_net][M]
-  const float *uniforms;      // injected stream for this slot, [M][N], or null
-  const NetDev *nets;
-  const int32_t *net_of;      // node -> network
-  const uint32_t *rank_limit; // [n_net] of this window, or null (no cap bites)
-  const float *p_tab;         // [n_net][tab_max+1]: P of c whole windows, or null
-  int32_t tab_max;
-  uint64_t seed;
-  int64_t NW;                 // N * W words
-  int32_t N, M, W, Mp, n_net, depth;
-  int32_t k;
-  int32_t ws_rel, we_rel;
-  float scale;
-  uint32_t slot;
-};
 
 __device__ __forceinline__ const uint64_t *ring_word(const StepArgs &a, int32_t n, int64_t w) {
   // X(k+1-n) for n >= 2 (n == 1 is the local infection word)
@@ -865,7 +820,6 @@ __global__ void slice_kernel(StepArgs a, int pass, const int32_t *out_rowptr, in
   if (pass == 0) row_nnz[i] = n;
 }
 
-thread_local std::string g_open_error;
 
 // cu:530-550 on the host.
 int host_class(int32_t v, int32_t upper, int32_t Ed, int32_t Id, int32_t Rd) {
@@ -879,106 +833,7 @@ int host_class(int32_t v, int32_t upper, int32_t Ed, int32_t Id, int32_t Rd) {
 
 }  // namespace
 
-// One network of a context, on the host.
-struct NetHost {
-  rp::Network net;
-  rp::Plan plan;
-  rp::Slices slices;
-  float p0 = 0.0f, beta = 0.0f, log_base = 0.0f;
-  int32_t Ed = 0, Id = 0, Rd = 0, upper = 0, medium = 0, lower = 0;
-  uint32_t stream = 0;
-  int32_t node_off = 0;
-  // timer tables
-  std::vector<int32_t> lut_new, lut_init;  // c(n), n = 0..T
-  int32_t n_on = kNever, n_off = kNever, n_s = kNever;
-  int32_t m_on = kNever, m_off = kNever, m_s = kNever;
-  Segs segs{};
-  std::vector<int8_t> class_init;  // class of c_init(j), j = 0..T
-};
-
-struct replay_ctx {
-  replay_params p{};
-  std::vector<NetHost> nets;
-  int32_t n_net = 0;
-  int32_t N = 0, M = 0, W = 0, Mp = 0, T = 0, depth = 1;
-  // merged plan and slices
-  std::vector<int64_t> iter_time;
-  std::vector<int32_t> iter_window;
-  std::vector<int64_t> win_ws, win_we;
-  std::vector<float> win_scale;
-  std::vector<int32_t> win_epoch;
-  std::vector<uint8_t> win_int;
-  std::vector<int64_t> epoch_t0, epoch_entry0;
-  std::vector<int32_t> epoch_maxrow;  // longest row of each epoch
-  int64_t n_entries = 0;
-  int32_t windows_per_epoch = 1;
-  bool any_cut = false;
-  int truncated = 0;
-  int64_t n_contacts = 0, nlists = 0;
-  int32_t variant = REPLAY_V_WORD;
-  hipStream_t stream = nullptr;
-  hipGraphExec_t graph = nullptr;
-  // device
-  int4 *d_entries = nullptr;
-  int32_t *d_rowptr = nullptr;
-  uint64_t *d_S = nullptr;
-  uint64_t *d_I[2] = {nullptr, nullptr};
-  uint64_t *d_X = nullptr;
-  uint64_t *d_Xinit = nullptr;
-  int32_t *d_entry = nullptr;
-  int32_t *d_inf = nullptr;
-  int32_t *d_P = nullptr;
-  int32_t *d_init_count = nullptr;
-  int32_t *d_counts = nullptr;
-  int8_t *d_class_init = nullptr;
-  Segs *d_segs = nullptr;
-  NetDev *d_nets = nullptr;
-  int32_t *d_net_of = nullptr;
-  uint32_t *d_rank_limit = nullptr;  // [n_windows][n_net] when a cap bites
-  float *d_p_tab = nullptr;          // [n_net][tab_max+1]
-  int32_t tab_max = 0;
-  float *d_uniforms = nullptr;
-  int cur = 0;
-  int32_t steps_done = 0;
-  int32_t counts_done = -1;
-  std::string err;
-};
-
 namespace {
-
-int fail(replay_ctx *ctx, int code, const std::string &msg) {
-  if (ctx)
-    ctx->err = msg;
-  else
-    g_open_error = msg;
-  return code;
-}
-
-#define HIP_OR_FAIL(ctx, call)                                                         \
-  do {                                                                                 \
-    hipError_t e_ = (call);                                                            \
-    if (e_ != hipSuccess)                                                              \
-      return fail((ctx), REPLAY_E_DEVICE, std::string(#call) + ": " + hipGetErrorString(e_)); \
-  } while (0)
-
-template <typename T>
-int dev_alloc(replay_ctx *ctx, T **ptr, size_t count) {
-  if (count == 0) count = 1;
-  hipError_t e = hipMalloc(reinterpret_cast<void **>(ptr), count * sizeof(T));
-  if (e != hipSuccess)
-    return fail(ctx, REPLAY_E_NOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
-  return REPLAY_OK;
-}
-
-template <typename T>
-int upload(replay_ctx *ctx, T **ptr, const T *src, size_t count) {
-  int rc = dev_alloc(ctx, ptr, count);
-  if (rc) return rc;
-  if (count)
-    HIP_OR_FAIL(ctx, hipMemcpyAsync(*ptr, src, count * sizeof(T), hipMemcpyHostToDevice,
-                                    ctx->stream));
-  return REPLAY_OK;
-}
 
 void release(replay_ctx *ctx) {
   if (!ctx) return;
@@ -1054,7 +909,9 @@ int plan_timers(int32_t T, float step, NetHost &nh, std::string &err) {
   return REPLAY_OK;
 }
 
-StepArgs step_args(replay_ctx *ctx, int32_t k) {
+}  // namespace
+
+StepArgs rp::step_args(replay_ctx *ctx, int32_t k) {
   const int32_t win = ctx->iter_window[k];
   const int e = ctx->win_epoch[win];
   const int64_t t0 = ctx->epoch_t0[e];
@@ -1093,6 +950,8 @@ StepArgs step_args(replay_ctx *ctx, int32_t k) {
   a.slot = static_cast<uint32_t>(k + 1);
   return a;
 }
+
+namespace {
 
 template <bool kInt, bool kInj>
 void launch_expo(const StepArgs &a, int variant, hipStream_t st) {

@@ -206,6 +206,40 @@ def config3_batched(seed: int = 0, n_networks: int = 64, M: int = 256, iteration
     return BatchWorkload("config3_batched_64nets", nets, p)
 
 
+@dataclass
+class MarginalWorkload:
+    """A contact network and iteration plan for the marginal-probability form,
+    with the parameter sets of one batch and the parameters of its target."""
+    name: str
+    graph: Workload         # contacts and the iteration / window plan
+    beta: np.ndarray        # (B,) per hour of contact
+    sigma: np.ndarray       # (B,) per hour
+    gamma: np.ndarray       # (B,) per hour
+    truth: tuple            # (beta, sigma, gamma) that generate the target curve
+
+
+def config4_marginal(seed: int = 0, n_nodes: int = 20000, block: int = 200, days: int = 10,
+                     iterations: int = 720, n_sets: int = 64) -> MarginalWorkload:
+    """Differentiable marginal SEIR: the village block recipe at 20k nodes
+    (100 blocks of 200, 90% of contacts within a block, diurnal starts,
+    LogNormal(ln 20 min, 1) durations) at the village contact rate over 10
+    days, 720 steps of 20 min, and 64 parameter sets log-uniform within +-50%
+    of (beta, sigma, gamma) = (0.03 per contact hour, 1/48 h, 1/120 h), the
+    parameters of the target curve."""
+    per_node_day = 2_000_000 / 10000 / 30  # contacts per node and day of the village recipe
+    g = config1_village(seed=seed, n_nodes=n_nodes, block=block,
+                        n_contacts=int(round(per_node_day * n_nodes * days)), M=64,
+                        iterations=iterations, days=days)
+    step = 1200
+    g = replace(g, name="config4_marginal_20k",
+                params=replace(g.params, step_size=float(step), static_network_duration=step))
+    rng = np.random.default_rng(seed + 1)
+    logu = lambda c: (c * np.exp(rng.uniform(np.log(0.5), np.log(1.5), size=n_sets))).astype(np.float32)
+    truth = (0.03, 1.0 / 48.0, 1.0 / 120.0)
+    return MarginalWorkload("config4_marginal_20k", g, logu(truth[0]), logu(truth[1]),
+                            logu(truth[2]), truth)
+
+
 def scaled(w: Workload, **kw) -> Workload:
     return replace(w, params=replace(w.params, **kw))
 

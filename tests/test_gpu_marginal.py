@@ -1,0 +1,149 @@
+"""HIP forward and hand-written adjoint of the marginal-probability SEIR form
+against the FP64 torch reference (oracle/marginal_ref.py), through the C ABI
+and the torch.autograd.Function.  Needs an MI355X.
+
+Tolerances (fp32 kernels against fp64): curves within 1e-5 of the node
+count; gradients within 1e-4 relative (norm over each parameter group)."""
+import numpy as np
+import pytest
+import torch
+
+from oracle import marginal_ref
+from replay_amd import Network, Params, Simulation
+from replay_amd.marginal import MarginalEngine, marginal_seir
+
+from test_gpu_parity import _random_case, _to_oracle_params
+
+pytestmark = pytest.mark.gpu
+
+DEV = "cuda:0"
+
+
+def _case(oracle, seed, n=48, T=36, step=1200.0, D=1200):
+    n, a, b, s, e = _random_case(seed, n=n, e=n * 10, span=T * int(step), long_frac=0.3)
+    p = Params(M=64, step_size=step, static_network_duration=D, iterations=T,
+               initial_infected=0.08, seed=seed)
+    sim = Simulation.from_contacts(n, a, b, s, e, p)
+    g = oracle.OracleGraph.from_contacts(n, a, b, s, e)
+    A = marginal_ref.dense_slices(g, _to_oracle_params(oracle, p), T)
+    return sim, A, p
+
+
+def _params(B, N, seed):
+    rng = np.random.default_rng(seed)
+    beta = rng.uniform(0.3, 2.0, B)
+    sigma = rng.uniform(0.05, 0.5, B)
+    gamma = rng.uniform(0.02, 0.3, B)
+    phi = rng.uniform(0.5, 1.5, N)
+    return [torch.tensor(x, dtype=torch.float64) for x in (beta, sigma, gamma, phi)]
+
+
+def _ref_grads(A, params, p0, dt_h, W):
+    xs = [t.clone().requires_grad_(True) for t in params]
+    C = marginal_ref.simulate(A, *xs, p0, dt_h)
+    L = (C * W).sum()
+    return C.detach(), torch.autograd.grad(L, xs)
+
+
+def _rel(a, b):
+    return float(torch.linalg.norm(a - b) / max(float(torch.linalg.norm(b)), 1e-30))
+
+
+@pytest.mark.parametrize("seed,B", [(0, 5), (1, 64), (2, 70)])
+def test_forward_and_adjoint_match_fp64(oracle, seed, B):
+    sim, A, p = _case(oracle, seed)
+    N, T = A.shape[1], A.shape[0]
+    params = _params(B, N, seed)
+    W = torch.tensor(np.random.default_rng(seed + 7).normal(size=(T + 1, B, 4)))
+    C_ref, g_ref = _ref_grads(A, params, p.initial_infected, p.step_size / 3600.0, W)
+    with sim:
+        eng = MarginalEngine(sim, B, checkpoint_every=8)
+        dev = [t.to(torch.float32).to(DEV) for t in params]
+        C = eng.forward(*dev)
+        g = eng.backward(W.to(torch.float32).to(DEV))
+        torch.cuda.synchronize()
+        eng.close()
+    C = C.cpu().double()
+    assert float((C - C_ref).abs().max()) <= 1e-5 * N
+    for name, got, ref in zip(("beta", "sigma", "gamma", "phi"), g, g_ref):
+        assert _rel(got.cpu().double(), ref) <= 1e-4, name
+
+
+def test_checkpoint_interval_does_not_change_results(oracle):
+    sim, A, p = _case(oracle, 3, T=30)
+    N, T = A.shape[1], A.shape[0]
+    B = 9
+    params = [t.to(torch.float32).to(DEV) for t in _params(B, N, 3)]
+    W = torch.randn(T + 1, B, 4, generator=torch.Generator().manual_seed(1)).to(DEV)
+    outs = []
+    with sim:
+        for every in (1, 7, 30, 1000):
+            eng = MarginalEngine(sim, B, checkpoint_every=every)
+            C = eng.forward(*params)
+            g = eng.backward(W)
+            torch.cuda.synchronize()
+            outs.append((C.cpu(), [x.cpu() for x in g]))
+            eng.close()
+    for C, g in outs[1:]:
+        assert torch.equal(C, outs[0][0])
+        for x, y in zip(g, outs[0][1]):
+            assert torch.equal(x, y)
+
+
+def test_autograd_function_mse_loss(oracle):
+    sim, A, p = _case(oracle, 4, T=24)
+    N, T = A.shape[1], A.shape[0]
+    B = 6
+    base = _params(B, N, 4)
+    target = marginal_ref.simulate(A, *_params(1, N, 40)[:3], base[3], p.initial_infected,
+                                   p.step_size / 3600.0)[:, 0, 2].to(torch.float32)
+    with sim:
+        eng = MarginalEngine(sim, B, checkpoint_every=5)
+        xs = [t.to(torch.float32).to(DEV).requires_grad_(True) for t in base]
+        C = marginal_seir(eng, *xs)
+        loss = ((C[:, :, 2] - target.to(DEV)[:, None]) ** 2).mean()
+        loss.backward()
+        torch.cuda.synchronize()
+        eng.close()
+    # the same gradients from the FP64 reference
+    rs = [t.clone().requires_grad_(True) for t in base]
+    Cr = marginal_ref.simulate(A, *rs, p.initial_infected, p.step_size / 3600.0)
+    lr = ((Cr[:, :, 2] - target.double()[:, None]) ** 2).mean()
+    lr.backward()
+    assert abs(loss.item() - lr.item()) <= 1e-5 * max(1.0, lr.item())
+    for x, r in zip(xs, rs):
+        assert _rel(x.grad.cpu().double(), r.grad) <= 1e-4
+
+
+def test_batched_context_per_network_seeds(oracle):
+    nets, As, p0s = [], [], []
+    for k, n in enumerate((30, 45)):
+        _, a, b, s, e = _random_case(50 + k, n=n, e=n * 8, span=20 * 1200, long_frac=0.3)
+        nets.append(Network(n_nodes=n, src=a, tgt=b, start=s, end=e,
+                            initial_infected=0.05 * (k + 1), infect_prob=0.2,
+                            exposed_duration=3600, infectious_duration=7200,
+                            resistant_duration=0, stream=k))
+        g = oracle.OracleGraph.from_contacts(n, a, b, s, e)
+        pk = Params(M=64, step_size=1200.0, static_network_duration=1200, iterations=20,
+                    initial_infected=0.05 * (k + 1))
+        As.append(marginal_ref.dense_slices(g, _to_oracle_params(oracle, pk), 20))
+        p0s.append(torch.full((n,), 0.05 * (k + 1), dtype=torch.float64))
+    T, N = 20, 75
+    A = torch.zeros(T, N, N, dtype=torch.float64)
+    A[:, :30, :30] = As[0]
+    A[:, 30:, 30:] = As[1]
+    p0 = torch.cat(p0s)
+    shared = Params(M=64, step_size=1200.0, static_network_duration=1200, iterations=T)
+    B = 4
+    params = _params(B, N, 8)
+    W = torch.tensor(np.random.default_rng(3).normal(size=(T + 1, B, 4)))
+    C_ref, g_ref = _ref_grads(A, params, p0, 1200 / 3600.0, W)
+    with Simulation.from_networks(nets, shared) as sim:
+        eng = MarginalEngine(sim, B, checkpoint_every=6)
+        C = eng.forward(*[t.to(torch.float32).to(DEV) for t in params])
+        g = eng.backward(W.to(torch.float32).to(DEV))
+        torch.cuda.synchronize()
+        eng.close()
+    assert float((C.cpu().double() - C_ref).abs().max()) <= 1e-5 * N
+    for got, ref in zip(g, g_ref):
+        assert _rel(got.cpu().double(), ref) <= 1e-4
