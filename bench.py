@@ -203,6 +203,87 @@ def bench_batched(args, w) -> dict:
     return out
 
 
+def bench_marginal(args) -> dict:
+    """Differentiable marginal form: forward + adjoint of an MSE loss against
+    a target curve, full workload size, timed per calibration step."""
+    import torch
+    from replay_amd import Simulation, workloads
+    from replay_amd.marginal import MarginalEngine, marginal_seir
+    w = workloads.config4_marginal()
+    g = w.graph
+    t0 = time.time()
+    sim = Simulation.from_contacts(g.n_nodes, g.src, g.tgt, g.start, g.end, g.params)
+    info = sim.info()
+    N, T, B = info["n_nodes"], info["iterations"], int(w.beta.size)
+    dev = "cuda:0"
+    eng = MarginalEngine(sim, B, checkpoint_every=32)
+    one = MarginalEngine(sim, 1, checkpoint_every=32)
+    phi = torch.ones(N, device=dev)
+    truth = [torch.tensor([v], dtype=torch.float32, device=dev) for v in w.truth]
+    target = one.forward(*truth, phi)[:, 0, 2].clone()
+    log(f"{w.name}: N={N} T={T} sets={B} contacts={info['n_contacts']} open {time.time() - t0:.1f}s")
+    xs = [torch.tensor(v, device=dev, requires_grad=True) for v in (w.beta, w.sigma, w.gamma)]
+    phi_p = torch.ones(N, device=dev, requires_grad=True)
+
+    def step():
+        for x in xs + [phi_p]:
+            x.grad = None
+        C = marginal_seir(eng, *xs, phi_p)
+        loss = ((C[:, :, 2] - target[:, None]) ** 2).mean()
+        loss.backward()
+        return loss
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(3)]
+    t0 = time.perf_counter()
+    fwd_ms, bwd_ms = [], []
+    for _ in range(args.steps):
+        for x in xs + [phi_p]:
+            x.grad = None
+        ev[0].record()
+        C = marginal_seir(eng, *xs, phi_p)
+        loss = ((C[:, :, 2] - target[:, None]) ** 2).mean()
+        ev[1].record()
+        loss.backward()
+        ev[2].record()
+        torch.cuda.synchronize()
+        fwd_ms.append(ev[0].elapsed_time(ev[1]))
+        bwd_ms.append(ev[1].elapsed_time(ev[2]))
+    wall = time.perf_counter() - t0
+    fwd = sum(fwd_ms) / len(fwd_ms)
+    bwd = sum(bwd_ms) / len(bwd_ms)
+    log(f"marginal: forward {fwd:.2f} ms, backward {bwd:.2f} ms per calibration step; "
+        f"loss {loss.item():.4g}")
+    # forward kernel: per step, four state planes read and written (fp32, 64 sets
+    # padded per node), the epoch's 16-byte entries and row pointers
+    Bp = (B + 63) // 64 * 64
+    ent = info["n_entries"] / max(info["n_epochs"], 1)
+    step_bytes = 32.0 * N * Bp + 16.0 * ent + 4.0 * (N + 1)
+    achieved = step_bytes / (fwd / 1e3 / T) / 1e9
+    eng.close()
+    one.close()
+    sim.close()
+    return {
+        "metric": "node*step*set/s (marginal SEIR forward + adjoint per calibration step)",
+        "value": N * T * B * args.steps / wall,
+        "ms_per_step": wall * 1e3 / args.steps,
+        "config": {"workload": w.name, "nodes": N, "iterations": T, "parameter_sets": B,
+                   "contacts": info["n_contacts"], "step_s": g.params.step_size,
+                   "checkpoint_every": 32, "forward_ms": fwd, "backward_ms": bwd,
+                   "final_loss": float(loss.item())},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "mf_forward_kernel (+ per-step reduction)",
+                     "bytes_per_launch": step_bytes,
+                     "us_per_launch": fwd * 1e3 / T,
+                     "bytes_definition": "per iteration: s, e, iota, r read and written as fp32 "
+                                         "for every node and padded parameter set; the slice "
+                                         "epoch's 16-byte contact entries and row pointers"},
+    }
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -218,6 +299,15 @@ def main() -> int:
         return 2
     from replay_amd import workloads
     t0 = time.time()
+    if args.workload == "config4":
+        res = bench_marginal(args)
+        out = {"metric": res["metric"], "value": res["value"], "unit": "node*step*set/s",
+               "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+               "ms_per_step": res["ms_per_step"], "higher_is_better": True, "scaling": "weak",
+               "vs_baseline": None, "dtype": "f32", "data": "synthetic", "config": res["config"],
+               "roofline": res["roofline"], "cpu_baseline": None}
+        print(json.dumps(out))
+        return 0
     if args.workload == "config3":
         w = workloads.config3_batched()
         log(f"generated {w.name} ({w.n_contacts} contacts) in {time.time() - t0:.1f}s")

@@ -231,8 +231,10 @@ int32_t replay_get_exposure(replay_ctx *ctx, int32_t k, float *out,
    curves and a hand-written adjoint (transposed-slice product plus the
    transition adjoints), with checkpoints every checkpoint_every iterations
    recomputed during the backward pass.  All data pointers of the forward and
-   backward calls are DEVICE pointers (fp32); stream is a hipStream_t (NULL:
-   the context's stream).  Results are deterministic. */
+   backward calls are DEVICE pointers (fp32); stream is the hipStream_t the
+   work is ordered on (NULL: HIP's default stream), so the caller's own
+   producers and consumers of those buffers on that stream need no extra
+   synchronisation.  Results are deterministic. */
 typedef struct replay_marginal replay_marginal;
 
 /* n_sets parameter sets over the context's network(s) and iterations. */

@@ -94,6 +94,27 @@ __global__ __launch_bounds__(kBlock) void mf_prep_kernel(const float *beta, cons
   if (t < N) phi[t] = phi_in[t];
 }
 
+// Adds the four compartment sums of the block's wavefronts in wavefront
+// order and writes them as the block's partial part[block][b][4].
+__device__ __forceinline__ void block_partial(const float (&sum)[4], int b, int Bp, float *part) {
+  __shared__ float red[kWavesPerBlock][kWave][4];
+  const int lane = threadIdx.x & (kWave - 1);
+  const int wv = threadIdx.x / kWave;
+#pragma unroll
+  for (int c = 0; c < 4; ++c) red[wv][lane][c] = sum[c];
+  __syncthreads();
+  if (wv == 0) {
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      float acc = red[0][lane][c];
+#pragma unroll
+      for (int w = 1; w < kWavesPerBlock; ++w) acc += red[w][lane][c];
+      part[(static_cast<size_t>(blockIdx.x) * Bp + b) * 4 + c] = acc;
+    }
+  }
+  __syncthreads();
+}
+
 // Initial state and its curve partials.
 __global__ __launch_bounds__(kBlock) void mf_init_kernel(float *st, const NetDev *nets,
                                                          const int32_t *net_of, float *part,
@@ -115,8 +136,7 @@ __global__ __launch_bounds__(kBlock) void mf_init_kernel(float *st, const NetDev
       sum[0] += 1.0f - p0;
       sum[2] += p0;
     }
-#pragma unroll
-    for (int c = 0; c < 4; ++c) part[(static_cast<size_t>(wave) * Bp + b) * 4 + c] = sum[c];
+    block_partial(sum, b, Bp, part);
   }
 }
 
@@ -151,19 +171,28 @@ __global__ __launch_bounds__(kBlock) void mf_forward_kernel(MfStep g, MfParams p
       sum[2] += i1;
       sum[3] += r1;
     }
-#pragma unroll
-    for (int q = 0; q < 4; ++q) part[(static_cast<size_t>(wave) * p.Bp + b) * 4 + q] = sum[q];
+    block_partial(sum, b, p.Bp, part);
   }
 }
 
-// curves[b][c] = sum over wavefronts of part[w][b][c], in wavefront order.
-__global__ void mf_reduce_kernel(const float *part, int n_waves, int Bp, int B, float *curves) {
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= B * 4) return;
+// curves[b][c] = sum over blocks of part[blk][b][c]: one block per output,
+// strided per-thread sums and a fixed-order tree, so the result does not
+// depend on scheduling.
+__global__ __launch_bounds__(kBlock) void mf_reduce_kernel(const float *part, int n_parts, int Bp,
+                                                           float *curves) {
+  __shared__ float red[kBlock];
+  const int t = blockIdx.x;  // b * 4 + c
   const int b = t >> 2, c = t & 3;
   float acc = 0.0f;
-  for (int w = 0; w < n_waves; ++w) acc += part[(static_cast<size_t>(w) * Bp + b) * 4 + c];
-  curves[t] = acc;
+  for (int w = threadIdx.x; w < n_parts; w += kBlock)
+    acc += part[(static_cast<size_t>(w) * Bp + b) * 4 + c];
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  for (int h = kBlock / 2; h > 0; h >>= 1) {
+    if (static_cast<int>(threadIdx.x) < h) red[threadIdx.x] += red[threadIdx.x + h];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) curves[t] = red[0];
 }
 
 __global__ void mf_zero_kernel(float *p, size_t n) {
@@ -274,7 +303,7 @@ struct replay_marginal {
   float *d_segE = nullptr;              // [every][N][Bp]
   float *d_adj[2] = {nullptr, nullptr}; // [4][N][Bp]
   float *d_G = nullptr;                 // [N][Bp]
-  float *d_part = nullptr;              // [n_waves][Bp][4]
+  float *d_part = nullptr;              // [n_blocks][Bp][4]
   float *d_pg = nullptr;                // [n_waves][Bp][3]
   float *d_phig = nullptr;              // [N]
   float *d_kb = nullptr, *d_ac = nullptr, *d_phi = nullptr;
@@ -293,9 +322,9 @@ MfParams mf_params(const replay_marginal *m) {
   return MfParams{m->d_kb, m->d_ac, m->d_phi, m->N, m->Bp};
 }
 
-hipStream_t mf_stream(const replay_marginal *m, void *stream) {
-  return stream ? static_cast<hipStream_t>(stream) : m->ctx->stream;
-}
+// The caller's stream, used as given: NULL is HIP's default (null) stream,
+// which is also what a torch process passes for its default stream.
+hipStream_t mf_stream(void *stream) { return static_cast<hipStream_t>(stream); }
 
 void zero(float *p, size_t n, hipStream_t st) {
   const unsigned blocks = static_cast<unsigned>(std::min<size_t>((n + 255) / 256, 4096));
@@ -351,18 +380,19 @@ int32_t replay_marginal_forward(replay_marginal *m, const float *beta, const flo
   if (!m || !beta || !sigma || !gamma || !phi || !curves) return REPLAY_E_ARG;
   replay_ctx *ctx = m->ctx;
   HIP_OR_FAIL(ctx, hipSetDevice(ctx->p.device));
-  const hipStream_t st = mf_stream(m, stream);
+  const hipStream_t st = mf_stream(stream);
   const size_t plane = static_cast<size_t>(m->N) * m->Bp;
   const unsigned blocks = static_cast<unsigned>(m->n_waves / kWavesPerBlock);
-  const unsigned red_blocks = static_cast<unsigned>((m->B * 4 + 255) / 256);
+  const unsigned red_blocks = static_cast<unsigned>(m->B * 4);
+  const int n_parts = m->n_waves / kWavesPerBlock;
   const int prep_n = std::max(m->Bp, m->N);
   hipLaunchKernelGGL(mf_prep_kernel, dim3((prep_n + kBlock - 1) / kBlock), dim3(kBlock), 0, st,
                      beta, sigma, gamma, phi, m->d_kb, m->d_ac, m->d_phi, m->B, m->Bp, m->N,
                      m->dt_h);
   hipLaunchKernelGGL(mf_init_kernel, dim3(blocks), dim3(kBlock), 0, st, m->d_st[0], ctx->d_nets,
                      ctx->d_net_of, m->d_part, m->N, m->Bp);
-  hipLaunchKernelGGL(mf_reduce_kernel, dim3(red_blocks), dim3(256), 0, st, m->d_part, m->n_waves,
-                     m->Bp, m->B, curves);
+  hipLaunchKernelGGL(mf_reduce_kernel, dim3(red_blocks), dim3(kBlock), 0, st, m->d_part, n_parts,
+                     m->Bp, curves);
   const MfParams p = mf_params(m);
   int cur = 0;
   for (int32_t k = 0; k < m->T; ++k) {
@@ -372,8 +402,8 @@ int32_t replay_marginal_forward(replay_marginal *m, const float *beta, const flo
                                       hipMemcpyDeviceToDevice, st));
     hipLaunchKernelGGL(mf_forward_kernel, dim3(blocks), dim3(kBlock), 0, st, mf_step(ctx, k), p,
                        m->d_st[cur], m->d_st[cur ^ 1], static_cast<float *>(nullptr), m->d_part);
-    hipLaunchKernelGGL(mf_reduce_kernel, dim3(red_blocks), dim3(256), 0, st, m->d_part,
-                       m->n_waves, m->Bp, m->B, curves + static_cast<size_t>(k + 1) * m->B * 4);
+    hipLaunchKernelGGL(mf_reduce_kernel, dim3(red_blocks), dim3(kBlock), 0, st, m->d_part, n_parts,
+                       m->Bp, curves + static_cast<size_t>(k + 1) * m->B * 4);
     cur ^= 1;
   }
   HIP_OR_FAIL(ctx, hipGetLastError());
@@ -389,7 +419,7 @@ int32_t replay_marginal_backward(replay_marginal *m, const float *grad_curves, f
   replay_ctx *ctx = m->ctx;
   if (!m->have_forward) return fail(ctx, REPLAY_E_STATE, "replay_marginal_backward before forward");
   HIP_OR_FAIL(ctx, hipSetDevice(ctx->p.device));
-  const hipStream_t st = mf_stream(m, stream);
+  const hipStream_t st = mf_stream(stream);
   const size_t plane = static_cast<size_t>(m->N) * m->Bp;
   const unsigned blocks = static_cast<unsigned>(m->n_waves / kWavesPerBlock);
   const MfParams p = mf_params(m);

@@ -147,3 +147,73 @@ def test_batched_context_per_network_seeds(oracle):
     assert float((C.cpu().double() - C_ref).abs().max()) <= 1e-5 * N
     for got, ref in zip(g, g_ref):
         assert _rel(got.cpu().double(), ref) <= 1e-4
+
+
+def test_calibration_steps_reduce_loss(oracle):
+    """Gradient descent on log-parameters fits a target curve made at other
+    parameters (the calibration use of the adjoint)."""
+    sim, A, p = _case(oracle, 6, n=60, T=40)
+    N = A.shape[1]
+    B = 4
+    phi = torch.ones(N, device=DEV)
+    with sim:
+        one = MarginalEngine(sim, 1, checkpoint_every=10)
+        truth = [torch.tensor([v], dtype=torch.float32, device=DEV) for v in (1.2, 0.3, 0.1)]
+        target = one.forward(*truth, phi)[:, 0, 2].clone()
+        eng = MarginalEngine(sim, B, checkpoint_every=10)
+        logs = [torch.log(torch.tensor(v, dtype=torch.float32, device=DEV)).requires_grad_(True)
+                for v in ([0.4, 0.6, 2.5, 3.0], [0.15, 0.6, 0.2, 0.5], [0.2, 0.05, 0.3, 0.15])]
+        opt = torch.optim.Adam(logs, lr=0.05)
+        losses = []
+        for _ in range(60):
+            opt.zero_grad()
+            C = marginal_seir(eng, *[torch.exp(x) for x in logs], phi)
+            loss = ((C[:, :, 2] - target[:, None]) ** 2).mean(0)  # per set
+            loss.sum().backward()
+            opt.step()
+            losses.append(loss.detach().cpu())
+        torch.cuda.synchronize()
+        eng.close()
+        one.close()
+    assert torch.all(losses[-1] < 0.2 * losses[0])
+
+
+def test_config4_full_size_directional_derivatives():
+    """Full workload size: the adjoint's directional derivatives against
+    central differences of the HIP forward (fp32, step 1e-3 relative)."""
+    from replay_amd import workloads
+    w = workloads.config4_marginal()
+    g = w.graph
+    with Simulation.from_contacts(g.n_nodes, g.src, g.tgt, g.start, g.end, g.params) as sim:
+        N = sim.info()["n_nodes"]
+        B = int(w.beta.size)
+        eng = MarginalEngine(sim, B, checkpoint_every=32)
+        one = MarginalEngine(sim, 1, checkpoint_every=32)
+        phi = torch.ones(N, device=DEV)
+        truth = [torch.tensor([v], dtype=torch.float32, device=DEV) for v in w.truth]
+        target = one.forward(*truth, phi)[:, 0, 2].double()
+        base = [torch.tensor(v, device=DEV) for v in (w.beta, w.sigma, w.gamma)] + [phi]
+
+        def loss_of(xs):
+            C = eng.forward(*xs)
+            return float(((C[:, :, 2].double() - target[:, None]) ** 2).mean())
+
+        xs = [x.clone().requires_grad_(True) for x in base]
+        C = marginal_seir(eng, *xs)
+        L = ((C[:, :, 2] - target.float()[:, None]) ** 2).mean()
+        L.backward()
+        grads = [x.grad.double() for x in xs]
+        gen = torch.Generator(device="cpu").manual_seed(5)
+        for k in range(4):
+            d = torch.randn(base[k].shape, generator=gen).to(DEV)
+            h = 1e-3
+            plus = [x.clone() for x in base]
+            minus = [x.clone() for x in base]
+            plus[k] = base[k] * (1 + h * d)
+            minus[k] = base[k] * (1 - h * d)
+            fd = (loss_of(plus) - loss_of(minus)) / (2 * h)
+            ad = float((grads[k] * base[k].double() * d.double()).sum())
+            assert abs(fd - ad) <= 2e-2 * abs(ad) + 1e-9 * abs(loss_of(base)), (k, fd, ad)
+        torch.cuda.synchronize()
+        eng.close()
+        one.close()
