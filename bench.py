@@ -62,11 +62,26 @@ def cpu_baseline(name, n_nodes, src, tgt, start, end, p, threads: int, M_s: int,
                       f"when nnz > 20000)"}
 
 
-def roofline(info: dict, per_launch_s: float) -> dict:
+def measured_traffic(workload: str):
+    """HBM bytes per step launch measured by rocprofv3 --pmc for this workload
+    (FETCH_SIZE and WRITE_SIZE, scaled by a copy of known size), from the
+    committed profile summary; None when there is none."""
+    path = os.path.join(ROOT, "profiles", f"traffic_{workload}.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        return json.load(f)
+
+
+def roofline(info: dict, per_launch_s: float, workload: str) -> dict:
     b = float(info["step_bytes"])
     achieved = b / per_launch_s / 1e9
+    tr = measured_traffic(workload)
     return {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": "step_kernel",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": tr["hbm_bytes_per_step"] if tr else None,
+            "traffic_source": tr["source"] if tr else None,
+            "kernel": "step (expo_kernel + update_kernel)",
             "bytes_per_launch": b, "us_per_launch": per_launch_s * 1e6,
             "bytes_definition": "per 64-replica word: susceptible word read+write, own "
                                 "infectious word read, next infectious word write, infection "
@@ -109,7 +124,7 @@ def bench_single(args, w) -> dict:
                    "window_s": w.params.static_network_duration,
                    "windows_per_epoch": info["windows_per_epoch"], "variant": info["variant"],
                    "final_recovered_mean": float(counts[-1, :, 2].mean())},
-        "roofline": roofline(info, per_launch),
+        "roofline": roofline(info, per_launch, w.name),
     }
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(w.name, w.n_nodes, w.src, w.tgt, w.start, w.end,
@@ -124,7 +139,7 @@ def bench_batched(args, w) -> dict:
     sp = w.params
     T, M = sp.iterations, sp.M
     t0 = time.time()
-    bp = Params(**{**sp.__dict__, "flags": replay_amd.F_GRAPH})
+    bp = Params(**{**sp.__dict__, "flags": 0 if args.eager else replay_amd.F_GRAPH})
     batch = Simulation.from_networks(w.networks, bp)
     variant = batch.autotune(min(T, 24))
     info = batch.info()
@@ -138,6 +153,19 @@ def bench_batched(args, w) -> dict:
     batched_value = N * M * T * args.steps / wall
     log(f"batched: {wall * 1e3 / args.steps:.2f} ms per run, {batched_value:.3e} r*n*s/s")
     batch.close()
+    out = {
+        "metric": "replica*node*steps/s (batched persistent form; plain per-network loop beside)",
+        "value": batched_value,
+        "ms_per_step": wall * 1e3 / args.steps,
+        "config": {"workload": w.name, "networks": len(w.networks), "nodes": N, "replicas": M,
+                   "iterations": T, "contacts": info["n_contacts"], "step_s": sp.step_size,
+                   "windows_per_epoch": info["windows_per_epoch"], "variant": variant,
+                   "hipgraph": not args.eager,
+                   "final_recovered_mean": float(bcounts[-1, :, :, 2].mean())},
+        "roofline": roofline(info, per_launch, w.name),
+    }
+    if args.no_plain:
+        return out
 
     # plain per-network launch loop: one context per network, eager launches
     t0 = time.time()
@@ -175,20 +203,10 @@ def bench_batched(args, w) -> dict:
         s.close()
     log(f"plain: {plain_wall * 1e3 / plain_steps:.2f} ms per run, {plain_value:.3e} r*n*s/s; "
         f"identical counts: {identical}")
-    out = {
-        "metric": "replica*node*steps/s (batched persistent form; plain per-network loop beside)",
-        "value": batched_value,
-        "ms_per_step": wall * 1e3 / args.steps,
-        "config": {"workload": w.name, "networks": len(w.networks), "nodes": N, "replicas": M,
-                   "iterations": T, "contacts": info["n_contacts"], "step_s": sp.step_size,
-                   "windows_per_epoch": info["windows_per_epoch"], "variant": variant,
-                   "hipgraph": True,
-                   "plain_value": plain_value, "plain_ms_per_step": plain_wall * 1e3 / plain_steps,
-                   "batched_over_plain": batched_value / plain_value,
-                   "batched_equals_plain": identical,
-                   "final_recovered_mean": float(bcounts[-1, :, :, 2].mean())},
-        "roofline": roofline(info, per_launch),
-    }
+    out["config"].update({"plain_value": plain_value,
+                          "plain_ms_per_step": plain_wall * 1e3 / plain_steps,
+                          "batched_over_plain": batched_value / plain_value,
+                          "batched_equals_plain": identical})
     if not args.no_cpu_baseline:
         nw = w.networks[0]
         p0 = Params(M=M, step_size=sp.step_size, iterations=T,
@@ -291,7 +309,23 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="config3")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-plain", action="store_true",
+                    help="batched workload only: skip the plain per-network loop")
+    ap.add_argument("--eager", action="store_true",
+                    help="launch every step eagerly (no hipGraph), e.g. under rocprofv3 --pmc")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="only copy MIB MiB device to device three times (counter calibration)")
     args = ap.parse_args()
+    if args.calibrate_copy:
+        import torch
+        n = args.calibrate_copy * (1 << 20) // 4
+        x = torch.ones(n, dtype=torch.float32, device="cuda:0")
+        y = torch.empty_like(x)
+        for _ in range(3):
+            y.copy_(x)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes": n * 4, "copies": 3}))
+        return 0
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (got --gpus {args.gpus}): replicas and networks "
               "are independent, so the path shards with no per-step exchange; multi-GPU runs "

@@ -273,21 +273,30 @@ __global__ __launch_bounds__(kBlock) void mf_spmm_adj_kernel(MfStep g, MfParams 
   }
 }
 
-// Final parameter gradients from the per-wavefront partials.
-__global__ void mf_grad_kernel(const float *pg, int n_waves, int Bp, int B, const float *ac,
-                               float dt_h, float *g_beta, float *g_sigma, float *g_gamma) {
-  const int b = blockIdx.x * blockDim.x + threadIdx.x;
-  if (b >= B) return;
-  float sb = 0.0f, sa = 0.0f, sc = 0.0f;
-  for (int w = 0; w < n_waves; ++w) {
-    const float *q = pg + (static_cast<size_t>(w) * Bp + b) * 3;
-    sb += q[0];
-    sa += q[1];
-    sc += q[2];
+// Final parameter gradients from the per-wavefront partials: one block per
+// (set, parameter), strided sums and a fixed-order tree.
+__global__ __launch_bounds__(kBlock) void mf_grad_kernel(const float *pg, int n_waves, int Bp,
+                                                         const float *ac, float dt_h,
+                                                         float *g_beta, float *g_sigma,
+                                                         float *g_gamma) {
+  __shared__ float red[kBlock];
+  const int b = blockIdx.x / 3, q = blockIdx.x % 3;
+  float acc = 0.0f;
+  for (int w = threadIdx.x; w < n_waves; w += kBlock)
+    acc += pg[(static_cast<size_t>(w) * Bp + b) * 3 + q];
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  for (int h = kBlock / 2; h > 0; h >>= 1) {
+    if (static_cast<int>(threadIdx.x) < h) red[threadIdx.x] += red[threadIdx.x + h];
+    __syncthreads();
   }
-  g_beta[b] = sb;
-  g_sigma[b] = sa * dt_h * (1.0f - ac[b]);
-  g_gamma[b] = sc * dt_h * (1.0f - ac[Bp + b]);
+  if (threadIdx.x != 0) return;
+  if (q == 0)
+    g_beta[b] = red[0];
+  else if (q == 1)
+    g_sigma[b] = red[0] * dt_h * (1.0f - ac[b]);
+  else
+    g_gamma[b] = red[0] * dt_h * (1.0f - ac[Bp + b]);
 }
 
 }  // namespace
@@ -449,8 +458,8 @@ int32_t replay_marginal_backward(replay_marginal *m, const float *grad_curves, f
       a ^= 1;
     }
   }
-  hipLaunchKernelGGL(mf_grad_kernel, dim3((m->B + 255) / 256), dim3(256), 0, st, m->d_pg,
-                     m->n_waves, m->Bp, m->B, m->d_ac, m->dt_h, grad_beta, grad_sigma, grad_gamma);
+  hipLaunchKernelGGL(mf_grad_kernel, dim3(3 * m->B), dim3(kBlock), 0, st, m->d_pg, m->n_waves,
+                     m->Bp, m->d_ac, m->dt_h, grad_beta, grad_sigma, grad_gamma);
   HIP_OR_FAIL(ctx, hipMemcpyAsync(grad_phi, m->d_phig, static_cast<size_t>(m->N) * sizeof(float),
                                   hipMemcpyDeviceToDevice, st));
   HIP_OR_FAIL(ctx, hipGetLastError());
