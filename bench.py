@@ -127,8 +127,11 @@ def bench_single(args, w) -> dict:
         "roofline": roofline(info, per_launch, w.name),
     }
     if not args.no_cpu_baseline:
+        # bounded samples of about 10-30 s of oracle work (replicas, iterations)
+        M_s, T_s = {"config0": (100, 168), "config1": (64, 240), "config2": (32, 6)}.get(
+            args.workload, (32, 16))
         out["cpu_baseline"] = cpu_baseline(w.name, w.n_nodes, w.src, w.tgt, w.start, w.end,
-                                           w.params, min(16, os.cpu_count() or 1), 64, 240)
+                                           w.params, min(16, os.cpu_count() or 1), M_s, T_s)
     return out
 
 

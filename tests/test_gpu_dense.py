@@ -1,0 +1,78 @@
+"""High-density contact stress (the config2 recipe: hub nodes with Pareto
+weights, long overlapping contacts, 5-minute steps, thousands of replicas).
+
+Reduced size: bit-exact against the oracle.  Reduced size also: replica r's
+trajectory does not depend on how many replicas run beside it (its uniforms
+are a function of (node, r, slot, stream) only).  Full size (50k nodes, 40M
+contacts, 4096 replicas, a 96-iteration prefix): every kernel variant gives
+identical counts and states, and each iteration's class counts sum to the
+node count.  Needs an MI355X."""
+import numpy as np
+import pytest
+
+import replay_amd
+from replay_amd import Params, Simulation, workloads
+
+from test_gpu_parity import _to_oracle_params
+
+pytestmark = pytest.mark.gpu
+
+
+def _reduced(M=192, T=72):
+    """The recipe at 1500 nodes (~170 simultaneous neighbours) with sojourns
+    shortened so that every transition happens within the horizon."""
+    w = workloads.config2_dense(n_nodes=1500, n_contacts=300_000, n_hubs=15, M=M, iterations=T,
+                                days=1)
+    w.params = Params(**{**w.params.__dict__, "initial_infected": 0.02, "infect_prob": 0.05,
+                         "exposed_duration": 3600, "infectious_duration": 3 * 3600,
+                         "resistant_duration": 2 * 3600})
+    return w
+
+
+def test_dense_reduced_matches_oracle(oracle):
+    w = _reduced()
+    p = w.params
+    with Simulation.from_contacts(w.n_nodes, w.src, w.tgt, w.start, w.end, p) as sim:
+        info = sim.info()
+        assert info["n_entries"] > 0
+        sim.run()
+        counts, state = sim.counts(), sim.state()
+    g = oracle.OracleGraph.from_contacts(w.n_nodes, w.src, w.tgt, w.start, w.end)
+    oc, ostate, _ = g.simulate(_to_oracle_params(oracle, p, threads=8), want_state=True)
+    np.testing.assert_array_equal(counts, oc)
+    np.testing.assert_array_equal(state.reshape(-1), ostate)
+
+
+def test_dense_replicas_independent_of_M():
+    w = _reduced(M=256, T=60)
+    out = {}
+    for M in (64, 256):
+        p = Params(**{**w.params.__dict__, "M": M})
+        with Simulation.from_contacts(w.n_nodes, w.src, w.tgt, w.start, w.end, p) as sim:
+            sim.run()
+            out[M] = (sim.counts(), sim.state())
+    np.testing.assert_array_equal(out[256][0][:, :64], out[64][0])
+    # states are column-major (idx = node + sim * N): replicas 0..63 come first
+    np.testing.assert_array_equal(out[256][1][: 64 * w.n_nodes], out[64][1])
+
+
+def test_dense_full_size_invariants():
+    w = workloads.config2_dense()
+    p = w.params
+    T = 96  # a prefix of the 4032 iterations keeps the test short
+    p = Params(**{**p.__dict__, "iterations": T})
+    with Simulation.from_contacts(w.n_nodes, w.src, w.tgt, w.start, w.end, p) as sim:
+        info = sim.info()
+        assert info["n_nodes"] == 50000 and info["M"] == 4096
+        results = []
+        for v in (replay_amd.V_NODE16, replay_amd.V_SLICED):
+            sim.set_variant(v)
+            sim.reset()
+            sim.run()
+            results.append((sim.counts(), sim.state()))
+    counts = results[0][0]
+    assert np.all(counts.sum(axis=2) == 50000)
+    assert counts[-1, :, :3].sum() > counts[0, :, :3].sum()  # the epidemic grew
+    for c, s in results[1:]:
+        np.testing.assert_array_equal(c, results[0][0])
+        np.testing.assert_array_equal(s, results[0][1])
