@@ -107,6 +107,7 @@ def bench_single(args, w) -> dict:
     from replay_amd import Simulation
     t0 = time.time()
     sim = Simulation.from_contacts(w.n_nodes, w.src, w.tgt, w.start, w.end, w.params)
+    sim.autotune(min(w.params.iterations, 48))
     info = sim.info()
     N, M, T = info["n_nodes"], info["M"], w.params.iterations
     log(f"{w.name}: N={N} M={M} T={T} contacts={info['n_contacts']} windows={info['n_windows']} "

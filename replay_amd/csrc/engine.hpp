@@ -57,6 +57,7 @@ struct StepArgs {
   const uint32_t *rank_limit; // [n_net] of this window, or null (no cap bites)
   const float *p_tab;         // [n_net][tab_max+1]: P of c whole windows, or null
   int32_t tab_max;
+  int32_t long_row;           // rows with more entries go to long_row_kernel
   uint64_t seed;
   int64_t NW;                 // N * W words
   int32_t N, M, W, Mp, n_net, depth;
@@ -98,7 +99,11 @@ struct replay_ctx {
   std::vector<int32_t> win_epoch;
   std::vector<uint8_t> win_int;
   std::vector<int64_t> epoch_t0, epoch_entry0;
-  std::vector<int32_t> epoch_maxrow;  // longest row of each epoch
+  std::vector<int32_t> epoch_maxrow;  // longest row of each epoch that is not long
+  // long rows: epoch e lists long_off[e] .. long_off[e+1] in d_long_rows,
+  // the first long_n_int[e] of them with exact integer sums
+  std::vector<int64_t> long_off;
+  std::vector<int32_t> long_n_int;
   int64_t n_entries = 0;
   int32_t windows_per_epoch = 1;
   bool any_cut = false;
@@ -125,6 +130,7 @@ struct replay_ctx {
   int32_t *d_net_of = nullptr;
   uint32_t *d_rank_limit = nullptr;  // [n_windows][n_net] when a cap bites
   float *d_p_tab = nullptr;          // [n_net][tab_max+1]
+  int32_t *d_long_rows = nullptr;    // long rows of every epoch
   int32_t tab_max = 0;
   float *d_uniforms = nullptr;
   int cur = 0;

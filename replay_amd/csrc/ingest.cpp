@@ -598,22 +598,32 @@ int build_slices(const Network &net, Plan &plan, int32_t windows_per_epoch,
     }
   }
   // Columns ascending within a row, rank order kept among equal columns.
+  // Integer sums are exact while a row's weights add up to less than 2^24:
+  // the bound of short rows decides each window's mode, long rows carry
+  // their own flag.
   int64_t max_row = 0;
   std::vector<int64_t> row_bound(static_cast<size_t>(n_epochs), 0);
   const int64_t D = we[0] - ws[0];
+  out.long_off.assign(static_cast<size_t>(n_epochs) + 1, 0);
   for (int32_t e = 0; e < n_epochs; ++e) {
     const int32_t *rp = &out.rowptr[static_cast<size_t>(e) * (N + 1)];
     Entry *base = out.entries.data() + out.epoch_entry0[e];
     for (int64_t r = 0; r < N; ++r) {
       Entry *b = base + rp[r], *x = base + rp[r + 1];
       int64_t len = x - b;
-      max_row = std::max(max_row, len);
       if (len > 1)
         std::stable_sort(b, x, [](const Entry &u, const Entry &v) { return u.col < v.col; });
       int64_t bound = 0;
       for (Entry *q = b; q < x; ++q) bound += std::min<int64_t>(q->te - q->ts, D);
-      row_bound[e] = std::max(row_bound[e], bound);
+      if (len > kLongRow) {
+        out.long_row.push_back(static_cast<int32_t>(r));
+        out.long_int.push_back(bound < (int64_t(1) << 24) ? 1 : 0);
+      } else {
+        max_row = std::max(max_row, len);
+        row_bound[e] = std::max(row_bound[e], bound);
+      }
     }
+    out.long_off[static_cast<size_t>(e) + 1] = static_cast<int64_t>(out.long_row.size());
   }
   out.max_row_entries = max_row;
   for (auto &w : plan.windows)

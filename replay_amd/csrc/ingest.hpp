@@ -58,13 +58,23 @@ struct Entry {
   uint32_t rank;
 };
 
+// Rows with more entries than this in an epoch are "long": the step kernels
+// skip them and a long-row kernel gives each of their 64-replica words its
+// own wavefront, so a hub's contact list does not serialise on one wavefront.
+constexpr int32_t kLongRow = 2048;
+
 struct Slices {
   int32_t windows_per_epoch = 1;
   std::vector<int64_t> epoch_t0;       // base time of each epoch
   std::vector<int64_t> epoch_entry0;   // first entry of each epoch
   std::vector<int32_t> rowptr;         // (n_epochs) x (N+1), epoch-relative
   std::vector<Entry> entries;
-  int64_t max_row_entries = 0;
+  int64_t max_row_entries = 0;         // longest row that is not long
+  // long rows of each epoch: rows long_row[long_off[e] .. long_off[e+1]),
+  // long_int = 1 where the row's weight bound allows exact integer sums
+  std::vector<int64_t> long_off;
+  std::vector<int32_t> long_row;
+  std::vector<uint8_t> long_int;
 };
 
 int parse_file(const char *path, Network &net, std::string &err);

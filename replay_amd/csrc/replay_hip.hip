@@ -169,15 +169,22 @@ __device__ __forceinline__ uint32_t column_count(uint64_t x, int lane) {
 // The wavefront loads 64 contact entries at a time (one per lane, coalesced)
 // together with the neighbours' infectious words of the range, then takes
 // the entries in order, broadcasting each one's overlap and words from its
-// lane: two memory round trips per 64 entries.  Integer mode adds overlaps
-// into exact integer sums; float mode merges the contributions fl(ov * scale)
-// of one column in rank order and adds columns in ascending order
-// (cu:299-309, Eigen SparseDenseProduct.h:64-70).
+// lane.  For WT <= 4 the loads are software-pipelined: the entries of the
+// chunk after next and the words of the next chunk are in flight while a
+// chunk is reduced, so a long row does not pay two memory round trips per
+// chunk.  Integer mode adds overlaps into exact integer sums; float mode
+// merges the contributions fl(ov * scale) of one column in rank order and
+// adds columns in ascending order (cu:299-309, Eigen
+// SparseDenseProduct.h:64-70), visiting only entries that overlap the window
+// with a nonzero word: a column's entries share one neighbour, so a column is
+// skipped whole or not at all, and adding nothing leaves the fp32 sum as it is.
 template <bool kIntMode, int WT>
 __device__ __forceinline__ void group_exposure(const StepArgs &a, int e0, int e1,
                                                uint32_t rank_limit, int q0, int nq, int lane,
                                                const uint64_t (&ss)[WT], float (&E)[WT],
                                                uint32_t (&cnt)[WT], uint32_t (&part)[WT]) {
+  typedef int v4i __attribute__((ext_vector_type(4)));
+  constexpr bool kPipe = WT <= 4;
   uint32_t iacc[WT];
   float facc[WT];
 #pragma unroll
@@ -194,25 +201,41 @@ __device__ __forceinline__ void group_exposure(const StepArgs &a, int e0, int e1
   uint32_t cur_lo[WT], cur_hi[WT];
 #pragma unroll
   for (int q = 0; q < WT; ++q) cur_lo[q] = cur_hi[q] = 0;
-  for (int c0 = e0; c0 < e1; c0 += kWave) {
+  // streamed once per iteration: keep the entries out of the caches' way
+  auto load_entries = [&](int c0) -> v4i {
     const int e = c0 + lane;
-    const bool ev = e < e1;
-    int col = 0, ov = 0;
-    if (ev) {
-      // streamed once per iteration: keep it out of the caches' way
-      typedef int v4i __attribute__((ext_vector_type(4)));
-      const v4i en = __builtin_nontemporal_load(reinterpret_cast<const v4i *>(a.entries + e));
-      col = en.x;
-      ov = min(en.z, a.we_rel) - max(en.y, a.ws_rel);
-      if (!(ov > 0 && static_cast<uint32_t>(en.w) <= rank_limit)) ov = 0;
-    }
-    uint32_t mlo[WT], mhi[WT];
+    return e < e1 ? __builtin_nontemporal_load(reinterpret_cast<const v4i *>(a.entries + e))
+                  : v4i{0, 0, 0, 0};
+  };
+  auto gather = [&](int c0, const v4i &en, int &col, int &ov, uint32_t (&lo)[WT],
+                    uint32_t (&hi)[WT]) {
+    const bool ev = c0 + lane < e1;
+    col = ev ? en.x : 0;
+    ov = ev ? min(en.z, a.we_rel) - max(en.y, a.ws_rel) : 0;
+    if (!(ov > 0 && static_cast<uint32_t>(en.w) <= rank_limit)) ov = 0;
     const uint64_t *mrow = a.I_cur + static_cast<size_t>(col) * a.W + q0;
 #pragma unroll
     for (int q = 0; q < WT; ++q) {
-      const uint64_t m = (ev && ov > 0 && q < nq && ss[q] != 0ull) ? mrow[q] : 0ull;
-      mlo[q] = static_cast<uint32_t>(m);
-      mhi[q] = static_cast<uint32_t>(m >> 32);
+      const uint64_t m = (ov > 0 && q < nq && ss[q] != 0ull) ? mrow[q] : 0ull;
+      lo[q] = static_cast<uint32_t>(m);
+      hi[q] = static_cast<uint32_t>(m >> 32);
+    }
+  };
+  int col = 0, ov = 0;
+  uint32_t mlo[WT], mhi[WT];
+  v4i en_next = v4i{0, 0, 0, 0};
+  if (e0 < e1) {
+    const v4i en0 = load_entries(e0);
+    if (kPipe && e0 + kWave < e1) en_next = load_entries(e0 + kWave);
+    gather(e0, en0, col, ov, mlo, mhi);
+  }
+  for (int c0 = e0; c0 < e1; c0 += kWave) {
+    int ncol = 0, nov = 0;
+    uint32_t nlo[WT], nhi[WT];
+    if (kPipe && c0 + kWave < e1) {
+      const v4i en2 = c0 + 2 * kWave < e1 ? load_entries(c0 + 2 * kWave) : v4i{0, 0, 0, 0};
+      gather(c0 + kWave, en_next, ncol, nov, nlo, nhi);
+      en_next = en2;
     }
     if (kIntMode) {
       // Whole-window contacts: count, per replica lane, the contacts whose
@@ -249,12 +272,13 @@ __device__ __forceinline__ void group_exposure(const StepArgs &a, int e0, int e1
         part[q] += add;
         iacc[q] += static_cast<uint32_t>(L) * c;
       }
-      continue;
     }
-    const int n_here = min(kWave, e1 - c0);
-    for (int j = 0; j < n_here; ++j) {
-      const int ovj = __builtin_amdgcn_readlane(ov, j);
-      {
+    if (!kIntMode) {
+      uint64_t any = 0ull;
+#pragma unroll
+      for (int q = 0; q < WT; ++q) any |= (static_cast<uint64_t>(mhi[q]) << 32) | mlo[q];
+      for (uint64_t act = __ballot(any != 0ull); act != 0ull; act &= act - 1) {
+        const int j = __builtin_ctzll(act);
         const int colj = __builtin_amdgcn_readlane(col, j);
         if (colj != cur_col) {  // flush the finished column
           if (cur_any) {
@@ -268,17 +292,26 @@ __device__ __forceinline__ void group_exposure(const StepArgs &a, int e0, int e1
           cur_w = 0.0f;
           cur_any = false;
         }
-        if (ovj > 0) {
-          const float wc = static_cast<float>(ovj) * a.scale;
-          cur_w = cur_any ? cur_w + wc : wc;
-          cur_any = true;
+        const float wc = static_cast<float>(__builtin_amdgcn_readlane(ov, j)) * a.scale;
+        cur_w = cur_any ? cur_w + wc : wc;
+        cur_any = true;
 #pragma unroll
-          for (int q = 0; q < WT; ++q) {
-            cur_lo[q] = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(mlo[q]), j));
-            cur_hi[q] = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(mhi[q]), j));
-          }
+        for (int q = 0; q < WT; ++q) {
+          cur_lo[q] = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(mlo[q]), j));
+          cur_hi[q] = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(mhi[q]), j));
         }
       }
+    }
+    if (kPipe) {
+      col = ncol;
+      ov = nov;
+#pragma unroll
+      for (int q = 0; q < WT; ++q) {
+        mlo[q] = nlo[q];
+        mhi[q] = nhi[q];
+      }
+    } else if (c0 + kWave < e1) {
+      gather(c0 + kWave, load_entries(c0 + kWave), col, ov, mlo, mhi);
     }
   }
   if (!kIntMode && cur_any) {
@@ -347,6 +380,7 @@ __global__ __launch_bounds__(kBlock) void expo_kernel(StepArgs a) {
   uint64_t *X_k = a.X + static_cast<size_t>(a.k % a.depth) * a.NW + static_cast<size_t>(i) * a.W;
   const uint64_t *S = a.S + static_cast<size_t>(i) * a.W;
   const int e0 = a.rowptr[i], e1 = a.rowptr[i + 1];
+  if (e1 - e0 > a.long_row) return;  // long_row_kernel takes it
   const int net = a.net_of[i];
   const NetDev nd = a.nets[net];
   const uint32_t rlim = a.rank_limit ? a.rank_limit[net] : UINT32_MAX;
@@ -387,6 +421,44 @@ __global__ __launch_bounds__(kBlock) void expo_kernel(StepArgs a) {
       if (lane == q) mine = x[q];
     if (lane < nq) X_k[q0 + lane] = mine;
   }
+}
+
+// Phase 1 for long rows (more than long_row entries in the epoch, e.g. the
+// hubs of a power-law network): one wavefront per (listed row, 64-replica
+// word), so a hub's contact list is walked by W wavefronts in parallel
+// instead of one.  Integer mode only where the row's own weight bound is
+// below 2^24; otherwise the float path, which follows the reference order.
+template <bool kIntMode, bool kInjected>
+__global__ __launch_bounds__(kBlock) void long_row_kernel(StepArgs a, const int32_t *rows,
+                                                          int n_rows) {
+  const int lane = threadIdx.x & (kWave - 1);
+  const int wid = static_cast<int>(blockIdx.x) * kWavesPerBlock +
+                  __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
+  if (wid >= n_rows * a.W) return;
+  const int h = wid / a.W;
+  const int q = wid - h * a.W;
+  const int i = rows[h];
+  uint64_t *X_k = a.X + static_cast<size_t>(a.k % a.depth) * a.NW + static_cast<size_t>(i) * a.W;
+  const uint64_t ss[1] = {a.S[static_cast<size_t>(i) * a.W + q]};
+  uint64_t x = 0ull;
+  if (ss[0] != 0ull) {
+    const int e0 = a.rowptr[i], e1 = a.rowptr[i + 1];
+    const int net = a.net_of[i];
+    const NetDev nd = a.nets[net];
+    const uint32_t rlim = a.rank_limit ? a.rank_limit[net] : UINT32_MAX;
+    float E[1];
+    uint32_t cnt[1], part[1];
+    group_exposure<kIntMode, 1>(a, e0, e1, rlim, q, 1, lane, ss, E, cnt, part);
+    const float *tab = (kIntMode && a.p_tab) ? a.p_tab + static_cast<size_t>(net) * (a.tab_max + 1)
+                                             : nullptr;
+    const float P_tab = (tab && part[0] == 0u && cnt[0] <= static_cast<uint32_t>(a.tab_max))
+                            ? tab[cnt[0]]
+                            : -1.0f;
+    uint32_t pw[4] = {0u, 0u, 0u, 0u};
+    int pw_grp = -1;
+    x = draw_word<kInjected>(a, nd, net, i, q, lane, ss[0], E[0], P_tab, pw, pw_grp);
+  }
+  if (lane == 0) X_k[q] = x;
 }
 
 // Carry-save adder of three 64-replica words (Harley and Seal): per replica
@@ -452,6 +524,7 @@ __global__ __launch_bounds__(kBlock) void sliced_kernel(StepArgs a, int G) {
     net_l = a.net_of[i_l];
     if (a.rank_limit) rlim_l = a.rank_limit[net_l];
     ss_l = a.S[static_cast<size_t>(i_l) * W + q_l];
+    if (e1_l - e0_l > a.long_row) e1_l = e0_l = -1;  // long_row_kernel takes it
     const NetDev &nd = a.nets[net_l];
     logb_l = nd.log_base;
     off_l = nd.node_off;
@@ -536,6 +609,7 @@ __global__ __launch_bounds__(kBlock) void sliced_kernel(StepArgs a, int G) {
     const int i = base + g;
     if (i >= a.N) break;
     const int slot0 = g * W;
+    if (__builtin_amdgcn_readlane(e0_l, slot0) < 0) continue;  // a long row
     const int net = __builtin_amdgcn_readlane(net_l, slot0);
     const float log_base = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(logb_l), slot0));
     const int node_off = __builtin_amdgcn_readlane(off_l, slot0);
@@ -782,8 +856,14 @@ __global__ __launch_bounds__(kBlock) void exposure_kernel(StepArgs a, float *out
   const int i = blockIdx.x * kWavesPerBlock + wave;
   if (i >= a.N) return;
   const uint32_t rlim = a.rank_limit ? a.rank_limit[a.net_of[i]] : UINT32_MAX;
-  const float E = row_exposure<kIntMode>(a.rowptr[i], a.rowptr[i + 1], a.entries, a.I_cur, a.W,
-                                         a.ws_rel, a.we_rel, rlim, a.scale, q, lane);
+  const int e0 = a.rowptr[i], e1 = a.rowptr[i + 1];
+  // the float order is the reference's and always exact; integer sums only
+  // where the window's short-row bound covers the row
+  const float E = (kIntMode && e1 - e0 <= a.long_row)
+                      ? row_exposure<true>(e0, e1, a.entries, a.I_cur, a.W, a.ws_rel, a.we_rel,
+                                           rlim, a.scale, q, lane)
+                      : row_exposure<false>(e0, e1, a.entries, a.I_cur, a.W, a.ws_rel,
+                                            a.we_rel, rlim, a.scale, q, lane);
   if (r < a.M) out[static_cast<size_t>(r) * a.N + i] = E;
 }
 
@@ -843,7 +923,8 @@ void release(replay_ctx *ctx) {
   void *bufs[] = {ctx->d_entries, ctx->d_rowptr, ctx->d_S,      ctx->d_I[0],        ctx->d_I[1],
                   ctx->d_X,       ctx->d_Xinit,  ctx->d_entry,  ctx->d_inf,         ctx->d_P,
                   ctx->d_init_count, ctx->d_counts, ctx->d_class_init, ctx->d_segs, ctx->d_nets,
-                  ctx->d_net_of,  ctx->d_rank_limit, ctx->d_uniforms, ctx->d_p_tab};
+                  ctx->d_net_of,  ctx->d_rank_limit, ctx->d_uniforms, ctx->d_p_tab,
+                  ctx->d_long_rows};
   for (void *b : bufs) (void)hipFree(b);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
   ctx->stream = nullptr;
@@ -935,6 +1016,7 @@ StepArgs rp::step_args(replay_ctx *ctx, int32_t k) {
   a.rank_limit = ctx->any_cut ? ctx->d_rank_limit + static_cast<size_t>(win) * ctx->n_net : nullptr;
   a.p_tab = ctx->d_p_tab;
   a.tab_max = ctx->tab_max;
+  a.long_row = rp::kLongRow;
   a.seed = ctx->p.seed;
   a.NW = static_cast<int64_t>(ctx->N) * ctx->W;
   a.N = ctx->N;
@@ -987,29 +1069,58 @@ void launch_sliced_nb(const StepArgs &a, bool injected, hipStream_t st) {
                        G);
 }
 
-// One iteration: exposure + draws per node, then the word flips.  nb_planes
-// > 0 selects the bit-sliced exposure kernel (integer windows only).
+// Long rows of the current epoch: rows[0 .. n_int) allow integer sums,
+// rows[n_int .. n_all) need the float order.
+struct LongRows {
+  const int32_t *rows;
+  int32_t n_int, n_all;
+};
+
+template <bool kInt, bool kInj>
+void launch_long_part(const StepArgs &a, const int32_t *rows, int32_t n, hipStream_t st) {
+  if (n <= 0) return;
+  const int64_t waves = static_cast<int64_t>(n) * a.W;
+  const dim3 grid(static_cast<unsigned>((waves + kWavesPerBlock - 1) / kWavesPerBlock));
+  hipLaunchKernelGGL((long_row_kernel<kInt, kInj>), grid, dim3(kBlock), 0, st, a, rows, n);
+}
+
+template <bool kInj>
+void launch_long(const StepArgs &a, bool int_mode, const LongRows &lr, hipStream_t st) {
+  if (int_mode) {
+    launch_long_part<true, kInj>(a, lr.rows, lr.n_int, st);
+    launch_long_part<false, kInj>(a, lr.rows + lr.n_int, lr.n_all - lr.n_int, st);
+  } else {
+    launch_long_part<false, kInj>(a, lr.rows, lr.n_all, st);
+  }
+}
+
+// One iteration: exposure + draws per node (long rows by long_row_kernel),
+// then the word flips.  nb_planes > 0 selects the bit-sliced exposure kernel
+// (integer windows only).
 void launch_step(const StepArgs &a, int variant, bool int_mode, bool injected, dim3 grid,
-                 hipStream_t st, int nb_planes = 0) {
-  if (variant == REPLAY_V_SLICED && int_mode && nb_planes > 0) {
+                 hipStream_t st, int nb_planes, const LongRows &lr) {
+  if (variant == REPLAY_V_SLICED && !(int_mode && nb_planes > 0))
+    variant = a.W <= 4 ? REPLAY_V_NODE4 : REPLAY_V_NODE16;
+  if (variant == REPLAY_V_SLICED) {
     if (nb_planes == 8)
       launch_sliced_nb<8>(a, injected, st);
     else if (nb_planes == 12)
       launch_sliced_nb<12>(a, injected, st);
     else
       launch_sliced_nb<16>(a, injected, st);
-    hipLaunchKernelGGL(update_kernel, grid, dim3(kBlock), 0, st, a);
-    return;
-  }
-  if (variant == REPLAY_V_SLICED) variant = a.W <= 4 ? REPLAY_V_NODE4 : REPLAY_V_NODE16;
-  if (int_mode && !injected)
+  } else if (int_mode && !injected) {
     launch_expo<true, false>(a, variant, st);
-  else if (int_mode)
+  } else if (int_mode) {
     launch_expo<true, true>(a, variant, st);
-  else if (!injected)
+  } else if (!injected) {
     launch_expo<false, false>(a, variant, st);
-  else
+  } else {
     launch_expo<false, true>(a, variant, st);
+  }
+  if (injected)
+    launch_long<true>(a, int_mode, lr, st);
+  else
+    launch_long<false>(a, int_mode, lr, st);
   hipLaunchKernelGGL(update_kernel, grid, dim3(kBlock), 0, st, a);
 }
 
@@ -1033,9 +1144,12 @@ int enqueue_steps(replay_ctx *ctx, int32_t steps) {
     const int32_t k = ctx->steps_done;
     StepArgs a = step_args(ctx, k);
     const int32_t win = ctx->iter_window[k];
+    const int32_t ep = ctx->win_epoch[win];
     const bool int_mode = ctx->win_int[win] != 0;
+    const LongRows lr{ctx->d_long_rows + ctx->long_off[ep], ctx->long_n_int[ep],
+                      static_cast<int32_t>(ctx->long_off[ep + 1] - ctx->long_off[ep])};
     launch_step(a, ctx->variant, int_mode, a.uniforms != nullptr, grid, ctx->stream,
-                sliced_planes(ctx->epoch_maxrow[ctx->win_epoch[win]]));
+                sliced_planes(ctx->epoch_maxrow[ep]), lr);
     HIP_OR_FAIL(ctx, hipGetLastError());
     ctx->cur ^= 1;
     ctx->steps_done = k + 1;
@@ -1240,8 +1354,30 @@ int finish_open(replay_ctx *ctx) {
     total += base;
     int32_t mr = 0;
     const int32_t *rpe = &rowptr[static_cast<size_t>(e) * (ctx->N + 1)];
-    for (int32_t r = 0; r < ctx->N; ++r) mr = std::max(mr, rpe[r + 1] - rpe[r]);
+    for (int32_t r = 0; r < ctx->N; ++r) {
+      const int32_t len = rpe[r + 1] - rpe[r];
+      if (len <= rp::kLongRow) mr = std::max(mr, len);
+    }
     ctx->epoch_maxrow[e] = mr;
+  }
+  // long rows of every epoch, integer-capable ones first
+  std::vector<int32_t> long_rows;
+  ctx->long_off.assign(static_cast<size_t>(n_ep) + 1, 0);
+  ctx->long_n_int.assign(static_cast<size_t>(n_ep), 0);
+  for (int32_t e = 0; e < n_ep; ++e) {
+    for (int pass = 0; pass < 2; ++pass) {
+      for (auto &nh : ctx->nets) {
+        const auto &sl = nh.slices;
+        if (sl.long_off.empty()) continue;
+        for (int64_t q = sl.long_off[e]; q < sl.long_off[e + 1]; ++q) {
+          if ((sl.long_int[static_cast<size_t>(q)] != 0) != (pass == 0)) continue;
+          long_rows.push_back(nh.node_off + sl.long_row[static_cast<size_t>(q)]);
+        }
+      }
+      if (pass == 0)
+        ctx->long_n_int[e] = static_cast<int32_t>(long_rows.size() - ctx->long_off[e]);
+    }
+    ctx->long_off[static_cast<size_t>(e) + 1] = static_cast<int64_t>(long_rows.size());
   }
   ctx->epoch_entry0[n_ep] = total;
   ctx->n_entries = total;
@@ -1290,6 +1426,7 @@ int finish_open(replay_ctx *ctx) {
       (rc = upload(ctx, &ctx->d_net_of, net_of.data(), net_of.size())) ||
       (rc = (ctx->any_cut ? upload(ctx, &ctx->d_rank_limit, limits.data(), limits.size())
                           : REPLAY_OK)) ||
+      (rc = upload(ctx, &ctx->d_long_rows, long_rows.data(), long_rows.size())) ||
       (rc = dev_alloc(ctx, &ctx->d_S, NW)) || (rc = dev_alloc(ctx, &ctx->d_I[0], NW)) ||
       (rc = dev_alloc(ctx, &ctx->d_I[1], NW)) ||
       (rc = dev_alloc(ctx, &ctx->d_X, NW * static_cast<size_t>(ctx->depth))) ||
@@ -1310,7 +1447,8 @@ int finish_open(replay_ctx *ctx) {
                        ctx->d_p_tab, ctx->d_nets, ctx->n_net, ctx->tab_max, D);
     HIP_OR_FAIL(ctx, hipGetLastError());
   }
-  ctx->variant = ctx->W <= 4 ? REPLAY_V_NODE4 : REPLAY_V_NODE16;
+  // measured defaults: 4-word groups up to 1024 replicas, 16-word groups above
+  ctx->variant = ctx->W <= 16 ? REPLAY_V_NODE4 : REPLAY_V_NODE16;
   rc = do_reset(ctx);
   if (rc != REPLAY_OK) return rc;
   HIP_OR_FAIL(ctx, hipStreamSynchronize(ctx->stream));

@@ -76,3 +76,49 @@ def test_dense_full_size_invariants():
     for c, s in results[1:]:
         np.testing.assert_array_equal(c, results[0][0])
         np.testing.assert_array_equal(s, results[0][1])
+
+
+def _two_hubs(seed=0, leaves=(3000, 5000), T=24):
+    """Two star hubs with more entries than the long-row threshold: with
+    one-hour windows the 3000-leaf hub's weights stay below 2^24 seconds
+    (integer sums), the 5000-leaf hub's exceed it (float order); leaves also
+    meet each other at random."""
+    rng = np.random.default_rng(seed)
+    n = 2 + sum(leaves)
+    src, tgt, st, en = [], [], [], []
+    t0 = 946713600
+    first = 2
+    for h, L in enumerate(leaves):
+        leaf = np.arange(first, first + L)
+        first += L
+        s = t0 + rng.integers(-3600, 3 * 3600, L)  # nearly all leaves in touch all along,
+        d = rng.integers((T - 4) * 3600, (T + 2) * 3600, L)  # some ending inside a window
+        src.append(np.full(L, h)); tgt.append(leaf); st.append(s); en.append(s + d)
+    k = 20000
+    a = rng.integers(2, n, k)
+    b = rng.integers(2, n, k)
+    keep = a != b
+    s = t0 + rng.integers(0, T * 3600, k)
+    src.append(a[keep]); tgt.append(b[keep]); st.append(s[keep])
+    en.append(s[keep] + rng.integers(60, 4 * 3600, k)[keep])
+    cat = lambda xs, dt: np.concatenate(xs).astype(dt)
+    st = np.maximum(cat(st, np.int64), t0 - 7200)
+    return n, cat(src, np.int32), cat(tgt, np.int32), st, cat(en, np.int64)
+
+
+@pytest.mark.parametrize("variant", [None, replay_amd.V_SLICED, replay_amd.V_WORD])
+def test_long_rows_match_oracle(oracle, variant):
+    n, a, b, s, e = _two_hubs()
+    p = Params(M=128, step_size=3600.0, static_network_duration=3600, iterations=24,
+               initial_infected=0.05, infect_prob=0.2, exposed_duration=3600,
+               infectious_duration=6 * 3600, resistant_duration=10 ** 6, seed=3)
+    with Simulation.from_contacts(n, a, b, s, e, p) as sim:
+        if variant is not None:
+            sim.set_variant(variant)
+            sim.reset()
+        sim.run()
+        counts, state = sim.counts(), sim.state()
+    g = oracle.OracleGraph.from_contacts(n, a, b, s, e)
+    oc, ostate, _ = g.simulate(_to_oracle_params(oracle, p, threads=8), want_state=True)
+    np.testing.assert_array_equal(counts, oc)
+    np.testing.assert_array_equal(state.reshape(-1), ostate)
