@@ -40,6 +40,32 @@ int32_t rp_host_plan_open(int32_t n_node_list, int64_t n, const int32_t *src, co
   return 0;
 }
 
+int32_t rp_host_parse_line(const char *line, int64_t len, int32_t fast, int32_t *src,
+                           int32_t *tgt, int64_t *start, int64_t *end, int32_t *counted) {
+  int c = 0;
+  const int r = rp::parse_line(line, len, fast, src, tgt, start, end, &c);
+  *counted = c;
+  return r;
+}
+
+// Parses a graph file and ranks its contacts: out[k] = {start, end, a, b}
+// in rank order; returns the count, or a negative error code.
+int64_t rp_host_parse_file(const char *path, int64_t *out, int64_t cap) {
+  rp::Network net;
+  std::string err;
+  int rc = rp::parse_file(path, net, err);
+  if (!rc) rc = rp::finalize(net, err);
+  if (rc) return rc;
+  const int64_t n = static_cast<int64_t>(net.contacts.size());
+  for (int64_t k = 0; k < n && k < cap; ++k) {
+    out[4 * k] = net.contacts[k].start;
+    out[4 * k + 1] = net.contacts[k].end;
+    out[4 * k + 2] = net.contacts[k].a;
+    out[4 * k + 3] = net.contacts[k].b;
+  }
+  return n;
+}
+
 int64_t rp_host_plan_nlists(rp_host_plan *h) { return h->net.nlists; }
 int32_t rp_host_plan_epochs(rp_host_plan *h) { return static_cast<int32_t>(h->slices.epoch_t0.size()); }
 int32_t rp_host_plan_int_mode(rp_host_plan *h, int32_t k) {

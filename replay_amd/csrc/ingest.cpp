@@ -104,6 +104,85 @@ int64_t parse_time(const char *s) {
   return static_cast<int64_t>(timegm(&t));
 }
 
+// Days from 1970-01-01 to year-month-01 of the proleptic Gregorian
+// calendar (H. Hinnant's days_from_civil), month 1..12.
+int64_t days_from_civil(int64_t y, int64_t m) {
+  y -= m <= 2;
+  const int64_t era = (y >= 0 ? y : y - 399) / 400;
+  const int64_t yoe = y - era * 400;
+  const int64_t doy = (153 * (m + (m > 2 ? -3 : 9)) + 2) / 5;
+  const int64_t doe = yoe * 365 + yoe / 4 - yoe / 100 + doy;
+  return era * 146097 + doe - 719468;
+}
+
+// Fixed-width digits at p (n of them); -1 when one is not a digit.
+inline int digits(const char *p, int n) {
+  int v = 0;
+  for (int i = 0; i < n; ++i) {
+    const unsigned d = static_cast<unsigned>(p[i] - '0');
+    if (d > 9) return -1;
+    v = v * 10 + static_cast<int>(d);
+  }
+  return v;
+}
+
+// "YYYY-MM-DDTHH:MM:SS" then a suffix of non-space characters other than
+// ',' and ';' up to `stop`.  Returns the end of the suffix, or nullptr when
+// the text is not in this form or the month is outside 1..12.  Within the
+// form, parse_time's sscanf reads the same six fields and timegm's
+// normalisation of day, hour, minute and second overflows is linear, so
+// the result equals parse_time's.
+const char *fast_time(const char *p, const char *end, char stop, int64_t *out) {
+  if (end - p < 19) return nullptr;
+  if (p[4] != '-' || p[7] != '-' || p[10] != 'T' || p[13] != ':' || p[16] != ':') return nullptr;
+  const int y = digits(p, 4), mo = digits(p + 5, 2), d = digits(p + 8, 2);
+  const int h = digits(p + 11, 2), mi = digits(p + 14, 2), se = digits(p + 17, 2);
+  if (y < 0 || mo < 1 || mo > 12 || d < 0 || h < 0 || mi < 0 || se < 0) return nullptr;
+  const char *q = p + 19;
+  if (q < end && static_cast<unsigned>(*q - '0') <= 9) return nullptr;  // a third seconds digit
+  while (q < end && *q != stop) {
+    if (*q == ' ' || *q == '\t' || *q == '\r' || *q == '\n' || *q == ',' || *q == ';') return nullptr;
+    ++q;
+  }
+  if (q >= end || q - p > 63) return nullptr;
+  *out = (days_from_civil(y, mo) + (d - 1)) * 86400 + h * 3600 + mi * 60 + se;
+  return q;
+}
+
+// A non-negative id of at most 9 digits immediately followed by ','.
+const char *fast_id(const char *p, const char *end, int32_t *out) {
+  int v = 0, n = 0;
+  while (p < end && static_cast<unsigned>(*p - '0') <= 9 && n < 10) {
+    v = v * 10 + (*p - '0');
+    ++p;
+    ++n;
+  }
+  if (n == 0 || n > 9 || p >= end || *p != ',') return nullptr;
+  *out = v;
+  return p + 1;
+}
+
+inline const char *skip_blank(const char *p, const char *end) {
+  while (p < end && (*p == ' ' || *p == '\t')) ++p;
+  return p;
+}
+
+// The canonical line "u, v, TIME, TIME;" (blanks allowed after the line
+// start and after each comma, none before a comma or the ';').  When it
+// matches, both sscanf patterns of scan_lines match with the same values.
+bool fast_line(const char *p, const char *end, int32_t *src, int32_t *tgt, int64_t *ts,
+               int64_t *te) {
+  p = skip_blank(p, end);
+  if (!(p = fast_id(p, end, src))) return false;
+  p = skip_blank(p, end);
+  if (!(p = fast_id(p, end, tgt))) return false;
+  p = skip_blank(p, end);
+  if (!(p = fast_time(p, end, ',', ts))) return false;
+  p = skip_blank(p + 1, end);
+  if (!(p = fast_time(p, end, ';', te))) return false;
+  return *ts != -1 && *te != -1;  // -1 is parse_time's failure value
+}
+
 struct LineResult {
   int64_t start;
   int64_t end;
@@ -112,6 +191,27 @@ struct LineResult {
   uint8_t counted;  // matches the counting pattern of tcm:135
   uint8_t valid;    // parsed (tcm:278) and both times convert (tcm:285-288)
 };
+
+// tcm:131-135 (counting pattern) and tcm:272-295 (parse, then both times)
+// on one NUL-terminated line.
+void slow_line(const char *line, LineResult &r) {
+  r.counted = 0;
+  r.valid = 0;
+  if (line[0] == '#') return;
+  int d1, d2;
+  char s[64], e[64];
+  if (std::sscanf(line, " %d, %d, %63[^,], %63[^;];", &d1, &d2, s, e) == 4) r.counted = 1;
+  int src, tgt;
+  if (std::sscanf(line, " %d , %d , %63[^,] , %63[^;\r\n] ;", &src, &tgt, s, e) != 4) return;
+  int64_t ts = parse_time(s);
+  int64_t te = parse_time(e);
+  if (ts == -1 || te == -1) return;
+  r.src = src;
+  r.tgt = tgt;
+  r.start = ts;
+  r.end = te;
+  r.valid = 1;
+}
 
 void scan_lines(const char *data, const std::vector<size_t> &line_start,
                 const std::vector<uint32_t> &line_len, size_t lo, size_t hi,
@@ -122,26 +222,16 @@ void scan_lines(const char *data, const std::vector<size_t> &line_start,
     r.counted = 0;
     r.valid = 0;
     const uint32_t len = line_len[k];
+    if (len > 0 && data[line_start[k]] != '#' &&
+        fast_line(data + line_start[k], data + line_start[k] + len, &r.src, &r.tgt, &r.start,
+                  &r.end)) {
+      r.counted = 1;
+      r.valid = 1;
+      continue;
+    }
     std::memcpy(buf.data(), data + line_start[k], len);
     buf[len] = '\0';
-    const char *line = buf.data();
-    if (line[0] == '#') continue;
-    int d1, d2;
-    char s[64], e[64];
-    if (std::sscanf(line, " %d, %d, %63[^,], %63[^;];", &d1, &d2, s, e) == 4)
-      r.counted = 1;
-    int src, tgt;
-    if (std::sscanf(line, " %d , %d , %63[^,] , %63[^;\r\n] ;", &src, &tgt, s,
-                    e) != 4)
-      continue;
-    int64_t ts = parse_time(s);
-    int64_t te = parse_time(e);
-    if (ts == -1 || te == -1) continue;
-    r.src = src;
-    r.tgt = tgt;
-    r.start = ts;
-    r.end = te;
-    r.valid = 1;
+    slow_line(buf.data(), r);
   }
 }
 
@@ -176,6 +266,25 @@ unsigned worker_count() {
 }
 
 }  // namespace
+
+int parse_line(const char *line, int64_t len, int fast, int32_t *src, int32_t *tgt,
+               int64_t *start, int64_t *end, int *counted) {
+  LineResult r{};
+  if (fast) {
+    if (len <= 0 || line[0] == '#' || !fast_line(line, line + len, &r.src, &r.tgt, &r.start, &r.end))
+      return -1;
+    r.counted = r.valid = 1;
+  } else {
+    std::string buf(line, static_cast<size_t>(len));
+    slow_line(buf.c_str(), r);
+  }
+  *src = r.src;
+  *tgt = r.tgt;
+  *start = r.start;
+  *end = r.end;
+  *counted = r.counted;
+  return r.valid;
+}
 
 int parse_file(const char *path, Network &net, std::string &err) {
   FILE *fp = std::fopen(path, "rb");

@@ -78,6 +78,11 @@ struct Slices {
 };
 
 int parse_file(const char *path, Network &net, std::string &err);
+// One contact line by the fast canonical-form parser (fast != 0; -1 when the
+// line is not in that form) or by the reference's sscanf patterns; returns
+// whether the line yields a contact, and sets *counted as tcm:131-135 would.
+int parse_line(const char *line, int64_t len, int fast, int32_t *src, int32_t *tgt,
+               int64_t *start, int64_t *end, int *counted);
 int from_contacts(int32_t n_list, const int64_t *node_ids, int64_t n,
                   const int32_t *src, const int32_t *tgt, const int64_t *start,
                   const int64_t *end, Network &net, std::string &err);
