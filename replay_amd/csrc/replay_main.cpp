@@ -37,36 +37,76 @@ int classify(int32_t v, int32_t upper, int32_t Ed, int32_t Id, int32_t Rd) {
   return 0;
 }
 
-// Buffered writer for the CSV rows.
+// Decimal digits of v, as printf's %d / %lld write them.
+inline char *put_int(char *p, long long v) {
+  unsigned long long u = static_cast<unsigned long long>(v);
+  if (v < 0) {
+    *p++ = '-';
+    u = 0ULL - u;
+  }
+  char tmp[20];
+  int n = 0;
+  do {
+    tmp[n++] = static_cast<char>('0' + u % 10);
+    u /= 10;
+  } while (u);
+  while (n) *p++ = tmp[--n];
+  return p;
+}
+
+// Buffered writer for the CSV rows: the bytes of "%lld,%d,%d,%d,%d,%d\n"
+// (cu:553-560) and "%lld,%d,%d,%s\n", formatted directly into the buffer.
 class Out {
  public:
-  explicit Out(FILE *f) : f_(f) { buf_.reserve(1 << 20); }
+  explicit Out(FILE *f) : f_(f) { buf_.reserve(kChunk + 256); }
   void row_summary(long long t, int sim, const int32_t *c4) {
-    char line[128];
-    int n = std::snprintf(line, sizeof line, "%lld,%d,%d,%d,%d,%d\n", t, sim, c4[0], c4[1], c4[2], c4[3]);
-    put(line, n);
+    char *p = room(128);
+    p = put_int(p, t);
+    *p++ = ',';
+    p = put_int(p, sim);
+    for (int c = 0; c < 4; ++c) {
+      *p++ = ',';
+      p = put_int(p, c4[c]);
+    }
+    *p++ = '\n';
+    commit(p);
   }
   void row_node(long long t, int sim, int node, const char *state) {
-    char line[128];
-    int n = std::snprintf(line, sizeof line, "%lld,%d,%d,%s\n", t, sim, node, state);
-    put(line, n);
+    const size_t ls = std::strlen(state);
+    char *p = room(64 + ls);
+    p = put_int(p, t);
+    *p++ = ',';
+    p = put_int(p, sim);
+    *p++ = ',';
+    p = put_int(p, node);
+    *p++ = ',';
+    std::memcpy(p, state, ls);
+    p += ls;
+    *p++ = '\n';
+    commit(p);
   }
   void flush() {
-    if (!buf_.empty()) std::fwrite(buf_.data(), 1, buf_.size(), f_);
-    buf_.clear();
+    if (used_) std::fwrite(buf_.data(), 1, used_, f_);
+    used_ = 0;
     std::fflush(f_);
   }
 
  private:
-  void put(const char *s, int n) {
-    buf_.append(s, static_cast<size_t>(n));
-    if (buf_.size() > (1u << 20)) {
-      std::fwrite(buf_.data(), 1, buf_.size(), f_);
-      buf_.clear();
+  static constexpr size_t kChunk = 1u << 20;
+  char *room(size_t n) {
+    if (used_ + n > buf_.size()) {
+      if (used_ >= kChunk) {
+        std::fwrite(buf_.data(), 1, used_, f_);
+        used_ = 0;
+      }
+      if (used_ + n > buf_.size()) buf_.resize(used_ + n + kChunk);
     }
+    return &buf_[used_];
   }
+  void commit(char *p) { used_ = static_cast<size_t>(p - buf_.data()); }
   FILE *f_;
-  std::string buf_;
+  std::vector<char> buf_;
+  size_t used_ = 0;
 };
 
 void usage(const char *argv0) {
