@@ -58,6 +58,13 @@ using namespace rp;
 
 namespace {
 
+#ifndef REPLAY_FEW_CONTACTS
+#define REPLAY_FEW_CONTACTS 2
+#endif
+// Chunk-words with at most this many whole-window contacts whose words are
+// nonzero are added one contact at a time instead of by the column count.
+constexpr int kFewContacts = REPLAY_FEW_CONTACTS;
+
 
 __device__ __forceinline__ const uint64_t *ring_word(const StepArgs &a, int32_t n, int64_t w) {
   // X(k+1-n) for n >= 2 (n == 1 is the local infection word)
@@ -252,7 +259,7 @@ __device__ __forceinline__ void group_exposure(const StepArgs &a, int e0, int e1
         // words without susceptible replicas need no exposure
         if (q >= nq || nz == 0ull || ss[q] == 0ull) continue;
         // few contributing contacts: add them one at a time
-        const bool few = __popcll(nz & ~partial) <= 6;
+        const bool few = __popcll(nz & ~partial) <= kFewContacts;
         const uint32_t c = (!few && (nz & ~partial)) ? column_count(full ? m : 0ull, lane) : 0u;
         uint32_t add = 0;
         for (uint64_t pm = few ? nz : (partial & nz); pm != 0ull; pm &= pm - 1) {

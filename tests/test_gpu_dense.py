@@ -122,3 +122,55 @@ def test_long_rows_match_oracle(oracle, variant):
     oc, ostate, _ = g.simulate(_to_oracle_params(oracle, p, threads=8), want_state=True)
     np.testing.assert_array_equal(counts, oc)
     np.testing.assert_array_equal(state.reshape(-1), ostate)
+
+
+def test_long_row_exposure_matches_oracle_order(oracle):
+    """Exposure of hub rows (integer-capable and beyond 2^24 seconds) equals
+    the sequential fp32 sum of the reference CSR bit for bit."""
+    import ctypes as C
+    n, a, b, s, e = _two_hubs(seed=2)
+    p = Params(M=96, step_size=3600.0, static_network_duration=3600, iterations=12,
+               initial_infected=0.3, infect_prob=0.2, exposed_duration=0,
+               infectious_duration=4 * 3600, resistant_duration=2 * 3600, seed=5)
+    g = oracle.OracleGraph.from_contacts(n, a, b, s, e)
+    op = _to_oracle_params(oracle, p)
+    P = lambda arr, t: arr.ctypes.data_as(C.POINTER(t))
+    with Simulation.from_contacts(n, a, b, s, e, p) as sim:
+        for k in (0, 4, 8):
+            state = sim.state()
+            got = sim.exposure(k)
+            lower, medium = p.resistant_duration, p.infectious_duration + p.resistant_duration
+            inf = ((state > lower) & (state <= medium)).astype(np.float32)
+            rp, col, val, _ = g.step_csr(op, k)
+            ref = np.zeros(n * p.M, np.float32)
+            oracle.lib().or_spmm(P(rp, C.c_int32), P(col, C.c_int32), P(val, C.c_float), len(col),
+                                 n, P(inf, C.c_float), P(ref, C.c_float), p.M, 1)
+            np.testing.assert_array_equal(got.view(np.int32), ref.view(np.int32))
+            sim.run(4)
+
+
+def test_batch_with_hub_network_equals_plain():
+    from replay_amd import Network
+    from test_gpu_parity import _random_case
+    nets = []
+    n, a, b, s, e = _two_hubs(seed=4, leaves=(2600, 4800), T=16)
+    nets.append(Network(n_nodes=n, src=a, tgt=b, start=s, end=e, initial_infected=0.03,
+                        infect_prob=0.15, exposed_duration=3600, infectious_duration=5 * 3600,
+                        resistant_duration=10 ** 6, stream=1))
+    n2, a2, b2, s2, e2 = _random_case(11, n=300, e=4000, span=16 * 3600, long_frac=0.3)
+    nets.append(Network(n_nodes=n2, src=a2, tgt=b2, start=s2, end=e2, initial_infected=0.1,
+                        infect_prob=0.3, exposed_duration=0, infectious_duration=3 * 3600,
+                        resistant_duration=0, stream=2))
+    shared = Params(M=160, step_size=3600.0, static_network_duration=3600, iterations=16, seed=9)
+    with Simulation.from_networks(nets, shared) as sim:
+        sim.run()
+        bc = sim.counts()
+    for k, nw in enumerate(nets):
+        pk = Params(M=160, step_size=3600.0, static_network_duration=3600, iterations=16, seed=9,
+                    initial_infected=nw.initial_infected, infect_prob=nw.infect_prob,
+                    exposed_duration=nw.exposed_duration,
+                    infectious_duration=nw.infectious_duration,
+                    resistant_duration=nw.resistant_duration, stream=nw.stream)
+        with Simulation.from_contacts(nw.n_nodes, nw.src, nw.tgt, nw.start, nw.end, pk) as s1:
+            s1.run()
+            np.testing.assert_array_equal(s1.counts(), bc[:, k])

@@ -217,3 +217,23 @@ def test_config4_full_size_directional_derivatives():
         torch.cuda.synchronize()
         eng.close()
         one.close()
+
+
+def test_marginal_error_paths(oracle):
+    import ctypes as C
+    from replay_amd import ReplayError, load_library
+    sim, A, p = _case(oracle, 9, T=8)
+    L = load_library()
+    with sim:
+        h = C.c_void_p()
+        assert L.replay_marginal_create(sim._ctx, 0, 4, C.byref(h)) != 0       # no sets
+        assert L.replay_marginal_create(sim._ctx, 3, 0, C.byref(h)) != 0       # no interval
+        assert L.replay_marginal_create(None, 3, 4, C.byref(h)) != 0           # no context
+        eng = MarginalEngine(sim, 3, checkpoint_every=4)
+        g = torch.zeros(9, 3, 4, device=DEV)
+        with pytest.raises(ReplayError):                                        # before forward
+            eng.backward(g)
+        with pytest.raises(ValueError):
+            eng.forward(torch.ones(2, device=DEV), torch.ones(3, device=DEV),
+                        torch.ones(3, device=DEV), torch.ones(A.shape[1], device=DEV))
+        eng.close()
