@@ -1,24 +1,27 @@
 """Benchmark of the HIP Monte Carlo SEIR step loop on one MI355X.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload config3|config1|...]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload config3|config1|config2|config4]
 
 A "step" is one full simulation of the workload: reset (initial draw) plus
-all T iterations of the step loop (fused exposure / draw / timer / counts
-kernel per iteration) over every replica.  Inputs (packed contact slices,
-state) are resident in HBM before timing; ingestion is excluded, per-step
-counts are included.  value = sum_k N_k * M * T * K / wall seconds, in
-replica*node*steps/s.
+all T iterations of the step loop (exposure / probability / draw kernel and
+timer-update kernel per iteration) over every replica, and the per-iteration
+SEIR counts.  Inputs (packed contact slices, state) are resident in HBM
+before timing; ingestion is excluded.  value = sum_k N_k * M * T * K / wall
+seconds, in replica*node*steps/s.
 
 config3 (default): 64 uneven networks.  The batched form (one context, one
 step launch for all networks, the T launches captured in a hipGraph, kernel
 variant picked by the autotuner) is timed against the plain per-network
-launch loop (one context per network, eager per-iteration launches) on the
-same GPU; both must give identical counts.
+launch loop (one context per network, eager per-iteration launches, one
+network at a time) on the same GPU; both must give identical counts.
+config1/config2: one network, autotuned variant.  config4: the
+differentiable marginal form, forward + adjoint per calibration step.
 
-Prints one JSON line (the driver's contract) with a roofline object for the
-step kernel (algorithmic bytes per launch / HIP-event time per launch) and a
-CPU baseline: the oracle (a C restatement of the reference --cpu-only loop)
-timed on a bounded sample of the same workload on the host cores.
+Prints one JSON line with a roofline object (algorithmic bytes per
+iteration over the HIP-event time per iteration; measured HBM traffic where
+a committed rocprofv3 summary holds it) and a CPU baseline: the oracle (a C
+restatement of the reference --cpu-only loop) timed on a bounded sample of
+the same workload on the host cores.
 """
 from __future__ import annotations
 
