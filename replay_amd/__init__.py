@@ -90,12 +90,24 @@ def library_path() -> str:
 
 
 def load_library() -> C.CDLL:
-    """Loads libreplay_hip.so; raises if it has not been built."""
+    """Loads libreplay_hip.so; raises if it has not been built.
+
+    A process must hold one HIP runtime.  PyTorch-ROCm ships its own
+    libamdhip64 / libhsa-runtime64, loaded by path; if this library were
+    loaded first, its dependency would resolve to the system ROCm copies and
+    torch would then load a second runtime that finds no device (and device
+    pointers would not be shared between the two).  So torch, when present,
+    is imported first: this library's libamdhip64.so.7 dependency then binds
+    to the runtime torch already holds."""
     global _lib
     if _lib is not None:
         return _lib
     if not os.path.exists(LIB_PATH):
         raise RuntimeError(f"{LIB_PATH} is missing: build it with `python -m replay_amd.build`")
+    try:
+        import torch  # noqa: F401  (one HIP runtime per process, see above)
+    except ImportError:
+        pass
     L = C.CDLL(LIB_PATH)
     P = C.POINTER
     ctx_p = C.c_void_p
