@@ -74,29 +74,53 @@ __device__ __forceinline__ float philox_uniform(uint64_t seed, uint32_t node, ui
   return u24(k == 0 ? w[0] : k == 1 ? w[1] : k == 2 ? w[2] : w[3]);
 }
 
+// A VGPR written with kBits where it is used.  gfx950's VOP3 encodings take
+// no literal, so a double coefficient other than 0.5, 1, 2 or 4 must sit in a
+// register pair; written by an instruction with side effects it is neither
+// hoisted out of loops nor kept live across the code between its uses (the
+// exponential's coefficients would otherwise hold about 20 VGPRs for the
+// whole of a kernel that calls it).
+template <uint32_t kBits>
+__device__ __forceinline__ uint32_t vgpr_const() {
+  uint32_t v;
+  asm volatile("v_mov_b32 %0, %1" : "=v"(v) : "i"(kBits));
+  return v;
+}
+
+template <uint64_t kBits>
+__device__ __forceinline__ double dconst() {
+  const uint64_t lo = vgpr_const<static_cast<uint32_t>(kBits)>();
+  const uint64_t hi = vgpr_const<static_cast<uint32_t>(kBits >> 32)>();
+  return __builtin_bit_cast(double, (hi << 32) | lo);
+}
+
+#define RP_DCONST(x) dconst<__builtin_bit_cast(uint64_t, static_cast<double>(x))>()
+
 __device__ __forceinline__ float rp_expf(float x) {
   if (x != x) return x;
   if (x > 88.72283935546875f) return __builtin_huge_valf();
   if (x < -104.0f) return 0.0f;
   const double xd = static_cast<double>(x);
-  const double kd = rint(xd * 1.4426950408889634074);
-  double r = fma(-kd, 6.93147180369123816490e-01, xd);
-  r = fma(-kd, 1.90821492927058770002e-10, r);
-  double p = 1.0 / 6227020800.0;
-  p = fma(p, r, 1.0 / 479001600.0);
-  p = fma(p, r, 1.0 / 39916800.0);
-  p = fma(p, r, 1.0 / 3628800.0);
-  p = fma(p, r, 1.0 / 362880.0);
-  p = fma(p, r, 1.0 / 40320.0);
-  p = fma(p, r, 1.0 / 5040.0);
-  p = fma(p, r, 1.0 / 720.0);
-  p = fma(p, r, 1.0 / 120.0);
-  p = fma(p, r, 1.0 / 24.0);
-  p = fma(p, r, 1.0 / 6.0);
+  const double kd = rint(xd * RP_DCONST(1.4426950408889634074));
+  double r = fma(-kd, RP_DCONST(6.93147180369123816490e-01), xd);
+  r = fma(-kd, RP_DCONST(1.90821492927058770002e-10), r);
+  double p = RP_DCONST(1.0 / 6227020800.0);
+  p = fma(p, r, RP_DCONST(1.0 / 479001600.0));
+  p = fma(p, r, RP_DCONST(1.0 / 39916800.0));
+  p = fma(p, r, RP_DCONST(1.0 / 3628800.0));
+  p = fma(p, r, RP_DCONST(1.0 / 362880.0));
+  p = fma(p, r, RP_DCONST(1.0 / 40320.0));
+  p = fma(p, r, RP_DCONST(1.0 / 5040.0));
+  p = fma(p, r, RP_DCONST(1.0 / 720.0));
+  p = fma(p, r, RP_DCONST(1.0 / 120.0));
+  p = fma(p, r, RP_DCONST(1.0 / 24.0));
+  p = fma(p, r, RP_DCONST(1.0 / 6.0));
   p = fma(p, r, 0.5);
   p = fma(p, r, 1.0);
   p = fma(p, r, 1.0);
   return static_cast<float>(ldexp(p, static_cast<int>(kd)));
 }
+
+#undef RP_DCONST
 
 }  // namespace rp

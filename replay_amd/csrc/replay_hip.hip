@@ -158,39 +158,80 @@ __device__ __forceinline__ void count_stage(uint64_t &x, int lane) {
 // lane r holds the number of rows with bit r set.  A butterfly over the lane
 // bits: at stage k, lanes differing in bit k exchange their words, each keeps
 // the fields (of width 2^k) whose column has its own bit k and adds the
-// partner's, so field widths double while the fields per lane halve.  In the
-// last stage each lane sends only the 32-bit field its partner keeps.
+// partner's, so field widths double while the fields per lane halve.
+//  - stages 0, 1: 1- and 2-bit fields are full, so each side is masked
+//    before the add;
+//  - stage 2: nibbles hold at most 4, so one 64-bit add sums every nibble
+//    pair without a carry, and the kept nibbles are masked after it;
+//  - stage 3: bytes hold at most 8; after the add, v_perm_b32 gathers the
+//    four kept bytes into 16-bit fields, columns 16j + (lane & 15) with
+//    j = 0, 2 in `a` and j = 1, 3 in `b`;
+//  - stage 4 (rows 2m <-> 2m+1): v_permlane16_swap trades the odd rows of
+//    `a` for the even rows of `b`, so an even-row lane holds `a` of itself
+//    and its partner (j = 0, 2) and an odd-row lane `b` of both (j = 1, 3);
+//  - stage 5 (lanes l <-> l^32): v_permlane32_swap gives every lane the
+//    words of l and l^32; their sum holds the totals of columns
+//    16j + (lane & 15) for j = (lane >> 4) & 1 (low half) and 2 + that
+//    (high half), and the lane takes the half its bit 5 names, which is
+//    column `lane`.
+// No LDS traffic: every exchange is a DPP or permlane move in the VALU.
 __device__ __forceinline__ uint32_t column_count(uint64_t x, int lane) {
   count_stage<0>(x, lane);
   count_stage<1>(x, lane);
-  count_stage<2>(x, lane);
-  count_stage<3>(x, lane);
-  count_stage<4>(x, lane);
-  const bool upper = lane & 32;
-  const uint32_t keep = upper ? static_cast<uint32_t>(x >> 32) : static_cast<uint32_t>(x);
-  const uint32_t give = upper ? static_cast<uint32_t>(x) : static_cast<uint32_t>(x >> 32);
-  return keep + static_cast<uint32_t>(__shfl_xor(static_cast<int>(give), 32));
+  {
+    const uint32_t ylo = row_xor<4>(static_cast<uint32_t>(x));
+    const uint32_t yhi = row_xor<4>(static_cast<uint32_t>(x >> 32));
+    const uint64_t t = x + ((static_cast<uint64_t>(yhi) << 32) | ylo);
+    x = (t >> ((lane & 4) ? 4 : 0)) & 0x0F0F0F0F0F0F0F0Full;
+  }
+  const uint32_t ylo = row_xor<8>(static_cast<uint32_t>(x));
+  const uint32_t yhi = row_xor<8>(static_cast<uint32_t>(x >> 32));
+  const uint64_t t = x + ((static_cast<uint64_t>(yhi) << 32) | ylo);
+  const uint32_t b3 = (static_cast<uint32_t>(lane) >> 3) & 1u;
+  // selector bytes 0-3 pick bytes of the low word, 4-7 of the high word,
+  // 0x0C a zero byte
+  const uint32_t a = __builtin_amdgcn_perm(static_cast<uint32_t>(t >> 32), static_cast<uint32_t>(t),
+                                           0x0C000C00u | ((4u + b3) << 16) | b3);
+  const uint32_t b = __builtin_amdgcn_perm(static_cast<uint32_t>(t >> 32), static_cast<uint32_t>(t),
+                                           0x0C000C00u | ((6u + b3) << 16) | (2u + b3));
+  const auto r16 = __builtin_amdgcn_permlane16_swap(a, b, false, false);
+  const uint32_t s = static_cast<uint32_t>(r16[0]) + static_cast<uint32_t>(r16[1]);
+  const auto r32 = __builtin_amdgcn_permlane32_swap(s, s, false, false);
+  const uint32_t u = static_cast<uint32_t>(r32[0]) + static_cast<uint32_t>(r32[1]);
+  return __builtin_amdgcn_ubfe(u, (lane & 32) ? 16u : 0u, 16u);
+}
+
+typedef int v4i __attribute__((ext_vector_type(4)));
+
+// Contact entries e .. e+63 of a row that ends at e1 (e < e1), one per lane;
+// lanes past the row re-read its last entry and are masked by the reader.
+// Streamed once per iteration: kept out of the caches' way.
+__device__ __forceinline__ v4i load_chunk(const int4 *entries, int e, int e1, int lane) {
+  return __builtin_nontemporal_load(reinterpret_cast<const v4i *>(entries + min(e + lane, e1 - 1)));
 }
 
 // Exposure of the words q0 .. q0+nq-1 (nq <= WT) of node i for one window.
-// The wavefront loads 64 contact entries at a time (one per lane, coalesced)
+// The wavefront takes 64 contact entries at a time (one per lane, coalesced)
 // together with the neighbours' infectious words of the range, then takes
 // the entries in order, broadcasting each one's overlap and words from its
-// lane.  For WT <= 4 the loads are software-pipelined: the entries of the
+// lane.  Rows of at most 128 entries (en0, en1: their entries, loaded by the
+// caller together with the node's other inputs) issue every neighbour-word
+// load of the row before waiting for any, and no load sits under an exec
+// mask: lanes without a counting contact read row 0 and are masked after the
+// loads.  Longer rows are software-pipelined for WT <= 4: the entries of the
 // chunk after next and the words of the next chunk are in flight while a
-// chunk is reduced, so a long row does not pay two memory round trips per
-// chunk.  Integer mode adds overlaps into exact integer sums; float mode
-// merges the contributions fl(ov * scale) of one column in rank order and
-// adds columns in ascending order (cu:299-309, Eigen
+// chunk is reduced.  Integer mode adds overlaps into exact integer sums;
+// float mode merges the contributions fl(ov * scale) of one column in rank
+// order and adds columns in ascending order (cu:299-309, Eigen
 // SparseDenseProduct.h:64-70), visiting only entries that overlap the window
 // with a nonzero word: a column's entries share one neighbour, so a column is
 // skipped whole or not at all, and adding nothing leaves the fp32 sum as it is.
 template <bool kIntMode, int WT>
 __device__ __forceinline__ void group_exposure(const StepArgs &a, int e0, int e1,
                                                uint32_t rank_limit, int q0, int nq, int lane,
-                                               const uint64_t (&ss)[WT], float (&E)[WT],
+                                               const uint64_t (&ss)[WT], const v4i &en0,
+                                               const v4i &en1, float (&E)[WT],
                                                uint32_t (&cnt)[WT], uint32_t (&part)[WT]) {
-  typedef int v4i __attribute__((ext_vector_type(4)));
   constexpr bool kPipe = WT <= 4;
   uint32_t iacc[WT];
   float facc[WT];
@@ -208,7 +249,28 @@ __device__ __forceinline__ void group_exposure(const StepArgs &a, int e0, int e1
   uint32_t cur_lo[WT], cur_hi[WT];
 #pragma unroll
   for (int q = 0; q < WT; ++q) cur_lo[q] = cur_hi[q] = 0;
-  // streamed once per iteration: keep the entries out of the caches' way
+  // overlap of this lane's entry of chunk c0 (0 when it does not count) and
+  // the column whose words it reads (row 0 when it does not count)
+  auto decode = [&](int c0, const v4i &en, int &col, int &ov) {
+    int o = min(en.z, a.we_rel) - max(en.y, a.ws_rel);
+    if (!(c0 + lane < e1 && o > 0 && static_cast<uint32_t>(en.w) <= rank_limit)) o = 0;
+    ov = o;
+    col = o > 0 ? en.x : 0;
+  };
+  auto fetch = [&](int col, uint64_t (&raw)[WT]) {
+    const uint64_t *mrow = a.I_cur + static_cast<size_t>(col) * a.W + q0;
+#pragma unroll
+    for (int q = 0; q < WT; ++q) raw[q] = mrow[q < nq ? q : nq - 1];
+  };
+  auto mask = [&](int ov, const uint64_t (&raw)[WT], uint32_t (&lo)[WT], uint32_t (&hi)[WT]) {
+#pragma unroll
+    for (int q = 0; q < WT; ++q) {
+      const uint64_t m = (ov > 0 && q < nq && ss[q] != 0ull) ? raw[q] : 0ull;
+      lo[q] = static_cast<uint32_t>(m);
+      hi[q] = static_cast<uint32_t>(m >> 32);
+    }
+  };
+  // loads of the long-row loop, under the lane's own conditions
   auto load_entries = [&](int c0) -> v4i {
     const int e = c0 + lane;
     return e < e1 ? __builtin_nontemporal_load(reinterpret_cast<const v4i *>(a.entries + e))
@@ -228,22 +290,9 @@ __device__ __forceinline__ void group_exposure(const StepArgs &a, int e0, int e1
       hi[q] = static_cast<uint32_t>(m >> 32);
     }
   };
-  int col = 0, ov = 0;
-  uint32_t mlo[WT], mhi[WT];
-  v4i en_next = v4i{0, 0, 0, 0};
-  if (e0 < e1) {
-    const v4i en0 = load_entries(e0);
-    if (kPipe && e0 + kWave < e1) en_next = load_entries(e0 + kWave);
-    gather(e0, en0, col, ov, mlo, mhi);
-  }
-  for (int c0 = e0; c0 < e1; c0 += kWave) {
-    int ncol = 0, nov = 0;
-    uint32_t nlo[WT], nhi[WT];
-    if (kPipe && c0 + kWave < e1) {
-      const v4i en2 = c0 + 2 * kWave < e1 ? load_entries(c0 + 2 * kWave) : v4i{0, 0, 0, 0};
-      gather(c0 + kWave, en_next, ncol, nov, nlo, nhi);
-      en_next = en2;
-    }
+  // one chunk of 64 entries: this lane's entry has column col, overlap ov
+  // and words (mlo, mhi), zero where the entry does not count
+  auto reduce = [&](int col, int ov, const uint32_t (&mlo)[WT], const uint32_t (&mhi)[WT]) {
     if (kIntMode) {
       // Whole-window contacts: count, per replica lane, the contacts whose
       // neighbour word has that replica's bit set (column counts of the
@@ -279,8 +328,7 @@ __device__ __forceinline__ void group_exposure(const StepArgs &a, int e0, int e1
         part[q] += add;
         iacc[q] += static_cast<uint32_t>(L) * c;
       }
-    }
-    if (!kIntMode) {
+    } else {
       uint64_t any = 0ull;
 #pragma unroll
       for (int q = 0; q < WT; ++q) any |= (static_cast<uint64_t>(mhi[q]) << 32) | mlo[q];
@@ -309,16 +357,55 @@ __device__ __forceinline__ void group_exposure(const StepArgs &a, int e0, int e1
         }
       }
     }
-    if (kPipe) {
-      col = ncol;
-      ov = nov;
-#pragma unroll
-      for (int q = 0; q < WT; ++q) {
-        mlo[q] = nlo[q];
-        mhi[q] = nhi[q];
+  };
+  if (e1 <= e0) {
+    // no contacts
+  } else if (e1 - e0 <= 2 * kWave) {
+    int colA, ovA, colB = 0, ovB = 0;
+    uint64_t rawA[WT], rawB[WT];
+    decode(e0, en0, colA, ovA);
+    fetch(colA, rawA);
+    const bool two = e1 - e0 > kWave;
+    if (two) {
+      decode(e0 + kWave, en1, colB, ovB);
+      fetch(colB, rawB);
+    }
+    uint32_t lo[WT], hi[WT];
+    mask(ovA, rawA, lo, hi);
+    reduce(colA, ovA, lo, hi);
+    if (two) {
+      mask(ovB, rawB, lo, hi);
+      reduce(colB, ovB, lo, hi);
+    }
+  } else {
+    int col = 0, ov = 0;
+    uint32_t mlo[WT], mhi[WT];
+    v4i en_next = v4i{0, 0, 0, 0};
+    {
+      const v4i first = load_entries(e0);
+      if (kPipe) en_next = load_entries(e0 + kWave);
+      gather(e0, first, col, ov, mlo, mhi);
+    }
+    for (int c0 = e0; c0 < e1; c0 += kWave) {
+      int ncol = 0, nov = 0;
+      uint32_t nlo[WT], nhi[WT];
+      if (kPipe && c0 + kWave < e1) {
+        const v4i en2 = c0 + 2 * kWave < e1 ? load_entries(c0 + 2 * kWave) : v4i{0, 0, 0, 0};
+        gather(c0 + kWave, en_next, ncol, nov, nlo, nhi);
+        en_next = en2;
       }
-    } else if (c0 + kWave < e1) {
-      gather(c0 + kWave, load_entries(c0 + kWave), col, ov, mlo, mhi);
+      reduce(col, ov, mlo, mhi);
+      if (kPipe) {
+        col = ncol;
+        ov = nov;
+#pragma unroll
+        for (int q = 0; q < WT; ++q) {
+          mlo[q] = nlo[q];
+          mhi[q] = nhi[q];
+        }
+      } else if (c0 + kWave < e1) {
+        gather(c0 + kWave, load_entries(c0 + kWave), col, ov, mlo, mhi);
+      }
     }
   }
   if (!kIntMode && cur_any) {
@@ -386,18 +473,31 @@ __global__ __launch_bounds__(kBlock) void expo_kernel(StepArgs a) {
   if (i >= a.N) return;
   uint64_t *X_k = a.X + static_cast<size_t>(a.k % a.depth) * a.NW + static_cast<size_t>(i) * a.W;
   const uint64_t *S = a.S + static_cast<size_t>(i) * a.W;
+  // The node's inputs are requested together before any is waited for: the
+  // row bounds and network (scalar), the first susceptible words and, once
+  // the bounds are in, the first two chunks of contact entries.
   const int e0 = a.rowptr[i], e1 = a.rowptr[i + 1];
-  if (e1 - e0 > a.long_row) return;  // long_row_kernel takes it
   const int net = a.net_of[i];
+  const uint64_t s_first = S[min(lane, min(WT, a.W) - 1)];
+  if (e1 - e0 > a.long_row) return;  // long_row_kernel takes it
+  v4i en0 = v4i{0, 0, 0, 0}, en1 = v4i{0, 0, 0, 0};
+  if (e1 > e0) {
+    en0 = load_chunk(a.entries, e0, e1, lane);
+    if (e1 - e0 > kWave) en1 = load_chunk(a.entries, e0 + kWave, e1, lane);
+  }
   const NetDev nd = a.nets[net];
   const uint32_t rlim = a.rank_limit ? a.rank_limit[net] : UINT32_MAX;
   for (int q0 = 0; q0 < a.W; q0 += WT) {
     const int nq = min(WT, a.W - q0);
+    // lane q holds susceptible word q0 + q (one load for the group)
+    const uint64_t sv = q0 == 0 ? s_first : S[q0 + min(lane, nq - 1)];
     uint64_t ss[WT];
     uint64_t any_sus = 0ull;
 #pragma unroll
     for (int q = 0; q < WT; ++q) {
-      ss[q] = q < nq ? S[q0 + q] : 0ull;
+      const uint32_t lo = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(sv), q));
+      const uint32_t hi = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(sv >> 32), q));
+      ss[q] = q < nq ? (static_cast<uint64_t>(hi) << 32) | lo : 0ull;
       any_sus |= ss[q];
     }
     uint64_t x[WT];
@@ -406,7 +506,7 @@ __global__ __launch_bounds__(kBlock) void expo_kernel(StepArgs a) {
     if (any_sus != 0ull && (e1 > e0 || kInjected)) {
       float E[WT];
       uint32_t cnt[WT], part[WT];
-      group_exposure<kIntMode, WT>(a, e0, e1, rlim, q0, nq, lane, ss, E, cnt, part);
+      group_exposure<kIntMode, WT>(a, e0, e1, rlim, q0, nq, lane, ss, en0, en1, E, cnt, part);
       const float *tab = (kIntMode && a.p_tab) ? a.p_tab + static_cast<size_t>(net) * (a.tab_max + 1)
                                                : nullptr;
       uint32_t pw[4] = {0u, 0u, 0u, 0u};
@@ -455,7 +555,8 @@ __global__ __launch_bounds__(kBlock) void long_row_kernel(StepArgs a, const int3
     const uint32_t rlim = a.rank_limit ? a.rank_limit[net] : UINT32_MAX;
     float E[1];
     uint32_t cnt[1], part[1];
-    group_exposure<kIntMode, 1>(a, e0, e1, rlim, q, 1, lane, ss, E, cnt, part);
+    const v4i none = v4i{0, 0, 0, 0};  // rows here are longer than two chunks
+    group_exposure<kIntMode, 1>(a, e0, e1, rlim, q, 1, lane, ss, none, none, E, cnt, part);
     const float *tab = (kIntMode && a.p_tab) ? a.p_tab + static_cast<size_t>(net) * (a.tab_max + 1)
                                              : nullptr;
     const float P_tab = (tab && part[0] == 0u && cnt[0] <= static_cast<uint32_t>(a.tab_max))
