@@ -214,13 +214,10 @@ __device__ __forceinline__ v4i load_chunk(const int4 *entries, int e, int e1, in
 // The wavefront takes 64 contact entries at a time (one per lane, coalesced)
 // together with the neighbours' infectious words of the range, then takes
 // the entries in order, broadcasting each one's overlap and words from its
-// lane.  Rows of at most 128 entries (en0, en1: their entries, loaded by the
-// caller together with the node's other inputs) issue every neighbour-word
-// load of the row before waiting for any, and no load sits under an exec
+// lane.  en0, en1 are the row's first two chunks of entries, loaded by the
+// caller together with the node's other inputs.  No load sits under an exec
 // mask: lanes without a counting contact read row 0 and are masked after the
-// loads.  Longer rows are software-pipelined for WT <= 4: the entries of the
-// chunk after next and the words of the next chunk are in flight while a
-// chunk is reduced.  Integer mode adds overlaps into exact integer sums;
+// loads.  Integer mode adds overlaps into exact integer sums;
 // float mode merges the contributions fl(ov * scale) of one column in rank
 // order and adds columns in ascending order (cu:299-309, Eigen
 // SparseDenseProduct.h:64-70), visiting only entries that overlap the window
@@ -230,17 +227,13 @@ template <bool kIntMode, int WT>
 __device__ __forceinline__ void group_exposure(const StepArgs &a, int e0, int e1,
                                                uint32_t rank_limit, int q0, int nq, int lane,
                                                const uint64_t (&ss)[WT], const v4i &en0,
-                                               const v4i &en1, float (&E)[WT],
-                                               uint32_t (&cnt)[WT], uint32_t (&part)[WT]) {
-  constexpr bool kPipe = WT <= 4;
+                                               const v4i &en1, float (&E)[WT]) {
   uint32_t iacc[WT];
   float facc[WT];
 #pragma unroll
   for (int q = 0; q < WT; ++q) {
     iacc[q] = 0;
     facc[q] = 0.0f;
-    cnt[q] = 0;
-    part[q] = 0;
   }
   // float mode: the column being merged, its weight and its words
   int cur_col = -1;
@@ -270,26 +263,6 @@ __device__ __forceinline__ void group_exposure(const StepArgs &a, int e0, int e1
       hi[q] = static_cast<uint32_t>(m >> 32);
     }
   };
-  // loads of the long-row loop, under the lane's own conditions
-  auto load_entries = [&](int c0) -> v4i {
-    const int e = c0 + lane;
-    return e < e1 ? __builtin_nontemporal_load(reinterpret_cast<const v4i *>(a.entries + e))
-                  : v4i{0, 0, 0, 0};
-  };
-  auto gather = [&](int c0, const v4i &en, int &col, int &ov, uint32_t (&lo)[WT],
-                    uint32_t (&hi)[WT]) {
-    const bool ev = c0 + lane < e1;
-    col = ev ? en.x : 0;
-    ov = ev ? min(en.z, a.we_rel) - max(en.y, a.ws_rel) : 0;
-    if (!(ov > 0 && static_cast<uint32_t>(en.w) <= rank_limit)) ov = 0;
-    const uint64_t *mrow = a.I_cur + static_cast<size_t>(col) * a.W + q0;
-#pragma unroll
-    for (int q = 0; q < WT; ++q) {
-      const uint64_t m = (ov > 0 && q < nq && ss[q] != 0ull) ? mrow[q] : 0ull;
-      lo[q] = static_cast<uint32_t>(m);
-      hi[q] = static_cast<uint32_t>(m >> 32);
-    }
-  };
   // one chunk of 64 entries: this lane's entry has column col, overlap ov
   // and words (mlo, mhi), zero where the entry does not count
   auto reduce = [&](int col, int ov, const uint32_t (&mlo)[WT], const uint32_t (&mhi)[WT]) {
@@ -310,22 +283,14 @@ __device__ __forceinline__ void group_exposure(const StepArgs &a, int e0, int e1
         // few contributing contacts: add them one at a time
         const bool few = __popcll(nz & ~partial) <= kFewContacts;
         const uint32_t c = (!few && (nz & ~partial)) ? column_count(full ? m : 0ull, lane) : 0u;
-        uint32_t add = 0;
         for (uint64_t pm = few ? nz : (partial & nz); pm != 0ull; pm &= pm - 1) {
           const int j = __builtin_ctzll(pm);
           const uint32_t lo = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(mlo[q]), j));
           const uint32_t hi = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(mhi[q]), j));
           const uint64_t mj = (static_cast<uint64_t>(hi) << 32) | lo;
           const uint32_t ovj = static_cast<uint32_t>(__builtin_amdgcn_readlane(ov, j));
-          const uint32_t bit = static_cast<uint32_t>((mj >> lane) & 1ull);
-          if (ovj == static_cast<uint32_t>(L))
-            cnt[q] += bit;  // whole-window contact taken one at a time
-          else
-            add += bit * ovj;
-          iacc[q] += bit * ovj;
+          iacc[q] += static_cast<uint32_t>((mj >> lane) & 1ull) * ovj;
         }
-        cnt[q] += c;
-        part[q] += add;
         iacc[q] += static_cast<uint32_t>(L) * c;
       }
     } else {
@@ -358,54 +323,36 @@ __device__ __forceinline__ void group_exposure(const StepArgs &a, int e0, int e1
       }
     }
   };
-  if (e1 <= e0) {
-    // no contacts
-  } else if (e1 - e0 <= 2 * kWave) {
+  // Chunks are taken in pairs for WT <= 4 (one at a time for wider groups,
+  // whose words take twice the registers): every neighbour-word load of the
+  // pair is issued before the first is waited for, then the pair is reduced
+  // and the next pair's entries are loaded.
+  constexpr bool kPair = WT <= 4;
+  v4i enA = en0, enB = en1;
+  for (int c0 = e0; c0 < e1;) {
     int colA, ovA, colB = 0, ovB = 0;
     uint64_t rawA[WT], rawB[WT];
-    decode(e0, en0, colA, ovA);
+    decode(c0, enA, colA, ovA);
     fetch(colA, rawA);
-    const bool two = e1 - e0 > kWave;
+    const bool two = kPair && c0 + kWave < e1;
     if (two) {
-      decode(e0 + kWave, en1, colB, ovB);
+      decode(c0 + kWave, enB, colB, ovB);
       fetch(colB, rawB);
     }
-    uint32_t lo[WT], hi[WT];
-    mask(ovA, rawA, lo, hi);
-    reduce(colA, ovA, lo, hi);
+    {
+      uint32_t lo[WT], hi[WT];
+      mask(ovA, rawA, lo, hi);
+      reduce(colA, ovA, lo, hi);
+    }
     if (two) {
+      uint32_t lo[WT], hi[WT];
       mask(ovB, rawB, lo, hi);
       reduce(colB, ovB, lo, hi);
     }
-  } else {
-    int col = 0, ov = 0;
-    uint32_t mlo[WT], mhi[WT];
-    v4i en_next = v4i{0, 0, 0, 0};
-    {
-      const v4i first = load_entries(e0);
-      if (kPipe) en_next = load_entries(e0 + kWave);
-      gather(e0, first, col, ov, mlo, mhi);
-    }
-    for (int c0 = e0; c0 < e1; c0 += kWave) {
-      int ncol = 0, nov = 0;
-      uint32_t nlo[WT], nhi[WT];
-      if (kPipe && c0 + kWave < e1) {
-        const v4i en2 = c0 + 2 * kWave < e1 ? load_entries(c0 + 2 * kWave) : v4i{0, 0, 0, 0};
-        gather(c0 + kWave, en_next, ncol, nov, nlo, nhi);
-        en_next = en2;
-      }
-      reduce(col, ov, mlo, mhi);
-      if (kPipe) {
-        col = ncol;
-        ov = nov;
-#pragma unroll
-        for (int q = 0; q < WT; ++q) {
-          mlo[q] = nlo[q];
-          mhi[q] = nhi[q];
-        }
-      } else if (c0 + kWave < e1) {
-        gather(c0 + kWave, load_entries(c0 + kWave), col, ov, mlo, mhi);
-      }
+    c0 += kPair ? 2 * kWave : kWave;
+    if (c0 < e1) {
+      enA = load_chunk(a.entries, c0, e1, lane);
+      if (kPair && c0 + kWave < e1) enB = load_chunk(a.entries, c0 + kWave, e1, lane);
     }
   }
   if (!kIntMode && cur_any) {
@@ -420,20 +367,20 @@ __device__ __forceinline__ void group_exposure(const StepArgs &a, int e0, int e1
 }
 
 // Draw of one word (word Q of node ii) given its lanes' exposure; records the
-// infections and returns their ballot.  P_tab >= 0 is the tabulated
-// probability of this lane's exposure (a whole number of windows).  The
-// Philox block serving words 4g .. 4g+3 is computed once per lane and kept
-// in pw / pw_grp.
+// infections and returns their ballot.  The probability is computed, not
+// looked up: a table read is one more dependent memory round trip per node,
+// which costs more than the exponential.  The Philox block serving words
+// 4g .. 4g+3 is computed once per lane and kept in pw / pw_grp.
 template <bool kInjected>
 __device__ __forceinline__ uint64_t draw_word(const StepArgs &a, const NetDev &nd, int net,
                                               int ii, int Q, int lane, uint64_t ss, float E,
-                                              float P_tab, uint32_t (&pw)[4], int &pw_grp) {
+                                              uint32_t (&pw)[4], int &pw_grp) {
   const int r = Q * kWave + lane;
   const bool sus = (ss >> lane) & 1ull;
   bool infect = false;
   // E == 0 gives P = 0 (or NaN), never above a uniform in [0, 1)
   if (sus && (kInjected || E > 0.0f)) {
-    const float P = P_tab >= 0.0f ? P_tab : 1.0f - rp::rp_expf(nd.log_base * (E / 3600.0f));
+    const float P = 1.0f - rp::rp_expf(nd.log_base * (E / 3600.0f));
     float u;
     if (kInjected) {
       u = a.uniforms[static_cast<size_t>(r) * a.N + ii];
@@ -483,7 +430,7 @@ __global__ __launch_bounds__(kBlock) void expo_kernel(StepArgs a) {
   v4i en0 = v4i{0, 0, 0, 0}, en1 = v4i{0, 0, 0, 0};
   if (e1 > e0) {
     en0 = load_chunk(a.entries, e0, e1, lane);
-    if (e1 - e0 > kWave) en1 = load_chunk(a.entries, e0 + kWave, e1, lane);
+    if (WT <= 4 && e1 - e0 > kWave) en1 = load_chunk(a.entries, e0 + kWave, e1, lane);
   }
   const NetDev nd = a.nets[net];
   const uint32_t rlim = a.rank_limit ? a.rank_limit[net] : UINT32_MAX;
@@ -505,21 +452,13 @@ __global__ __launch_bounds__(kBlock) void expo_kernel(StepArgs a) {
     for (int q = 0; q < WT; ++q) x[q] = 0ull;
     if (any_sus != 0ull && (e1 > e0 || kInjected)) {
       float E[WT];
-      uint32_t cnt[WT], part[WT];
-      group_exposure<kIntMode, WT>(a, e0, e1, rlim, q0, nq, lane, ss, en0, en1, E, cnt, part);
-      const float *tab = (kIntMode && a.p_tab) ? a.p_tab + static_cast<size_t>(net) * (a.tab_max + 1)
-                                               : nullptr;
+      group_exposure<kIntMode, WT>(a, e0, e1, rlim, q0, nq, lane, ss, en0, en1, E);
       uint32_t pw[4] = {0u, 0u, 0u, 0u};
       int pw_grp = -1;
 #pragma unroll
-      for (int q = 0; q < WT; ++q) {
-        if (q < nq && ss[q] != 0ull) {
-          const float P_tab = (tab && part[q] == 0u && cnt[q] <= static_cast<uint32_t>(a.tab_max))
-                                  ? tab[cnt[q]]
-                                  : -1.0f;
-          x[q] = draw_word<kInjected>(a, nd, net, i, q0 + q, lane, ss[q], E[q], P_tab, pw, pw_grp);
-        }
-      }
+      for (int q = 0; q < WT; ++q)
+        if (q < nq && ss[q] != 0ull)
+          x[q] = draw_word<kInjected>(a, nd, net, i, q0 + q, lane, ss[q], E[q], pw, pw_grp);
     }
     // lane q stores word q of the group
     uint64_t mine = 0ull;
@@ -554,17 +493,12 @@ __global__ __launch_bounds__(kBlock) void long_row_kernel(StepArgs a, const int3
     const NetDev nd = a.nets[net];
     const uint32_t rlim = a.rank_limit ? a.rank_limit[net] : UINT32_MAX;
     float E[1];
-    uint32_t cnt[1], part[1];
-    const v4i none = v4i{0, 0, 0, 0};  // rows here are longer than two chunks
-    group_exposure<kIntMode, 1>(a, e0, e1, rlim, q, 1, lane, ss, none, none, E, cnt, part);
-    const float *tab = (kIntMode && a.p_tab) ? a.p_tab + static_cast<size_t>(net) * (a.tab_max + 1)
-                                             : nullptr;
-    const float P_tab = (tab && part[0] == 0u && cnt[0] <= static_cast<uint32_t>(a.tab_max))
-                            ? tab[cnt[0]]
-                            : -1.0f;
+    const v4i en0 = load_chunk(a.entries, e0, e1, lane);  // rows here hold > 2 chunks
+    const v4i en1 = load_chunk(a.entries, e0 + kWave, e1, lane);
+    group_exposure<kIntMode, 1>(a, e0, e1, rlim, q, 1, lane, ss, en0, en1, E);
     uint32_t pw[4] = {0u, 0u, 0u, 0u};
     int pw_grp = -1;
-    x = draw_word<kInjected>(a, nd, net, i, q, lane, ss[0], E[0], P_tab, pw, pw_grp);
+    x = draw_word<kInjected>(a, nd, net, i, q, lane, ss[0], E[0], pw, pw_grp);
   }
   if (lane == 0) X_k[q] = x;
 }
