@@ -65,6 +65,10 @@ namespace {
 // nonzero are added one contact at a time instead of by the column count.
 constexpr int kFewContacts = REPLAY_FEW_CONTACTS;
 
+// Half-width of the band around a uniform inside which an infection draw
+// evaluates the reference's probability expression instead of its estimate.
+constexpr float kDecideMargin = 1.0e-5f;
+
 
 __device__ __forceinline__ const uint64_t *ring_word(const StepArgs &a, int32_t n, int64_t w) {
   // X(k+1-n) for n >= 2 (n == 1 is the local infection word)
@@ -369,7 +373,7 @@ __device__ __forceinline__ void group_exposure(const StepArgs &a, int e0, int e1
 // Draw of one word (word Q of node ii) given its lanes' exposure; records the
 // infections and returns their ballot.  The probability is computed, not
 // looked up: a table read is one more dependent memory round trip per node,
-// which costs more than the exponential.  The Philox block serving words
+// which costs more than the arithmetic.  The Philox block serving words
 // 4g .. 4g+3 is computed once per lane and kept in pw / pw_grp.
 template <bool kInjected>
 __device__ __forceinline__ uint64_t draw_word(const StepArgs &a, const NetDev &nd, int net,
@@ -380,7 +384,6 @@ __device__ __forceinline__ uint64_t draw_word(const StepArgs &a, const NetDev &n
   bool infect = false;
   // E == 0 gives P = 0 (or NaN), never above a uniform in [0, 1)
   if (sus && (kInjected || E > 0.0f)) {
-    const float P = 1.0f - rp::rp_expf(nd.log_base * (E / 3600.0f));
     float u;
     if (kInjected) {
       u = a.uniforms[static_cast<size_t>(r) * a.N + ii];
@@ -394,7 +397,21 @@ __device__ __forceinline__ uint64_t draw_word(const StepArgs &a, const NetDev &n
       const int k = Q & 3;
       u = rp::u24(k == 0 ? pw[0] : k == 1 ? pw[1] : k == 2 ? pw[2] : pw[3]);
     }
-    infect = P > u;
+    // The reference's P = 1 - expf(log_base * (E / 3600)) is needed only to
+    // compare it with u.  An estimate from the hardware base-2 exponential
+    // (v_exp_f32, within 1 ulp) of fl(log_base * fl(E * fl(1/3600))) differs
+    // from it by less than 5e-7: the relative error of the exponent, a few
+    // ulp, moves e^x by at most |x| e^x <= 1/e times that, and the two
+    // roundings of 1 - e^x add 1.2e-7.  Outside a band of kDecideMargin
+    // (20 times that bound) around u the estimate's comparison is the
+    // reference's; inside it, or when the estimate is NaN, the reference
+    // expression is evaluated.
+    const float x_est = nd.log_base * (E * (1.0f / 3600.0f));
+    const float P_est = 1.0f - __builtin_amdgcn_exp2f(x_est * 1.44269504088896341f);
+    if (fabsf(P_est - u) > kDecideMargin)
+      infect = P_est > u;
+    else
+      infect = 1.0f - rp::rp_expf(nd.log_base * (E / 3600.0f)) > u;
   }
   if (infect) {
     a.entry[static_cast<size_t>(ii) * a.Mp + r] = 2 * a.k + 1;

@@ -177,6 +177,63 @@ def test_injected_uniform_stream(oracle):
     np.testing.assert_array_equal(c2, counts2)
 
 
+@pytest.mark.parametrize("step,D", [(600.0, 600), (1000.0, 3600)])
+def test_draws_at_probability_ties(oracle, step, D):
+    """Uniforms equal to the reference probability of the exposures that
+    occur, and one float below it: every such draw sits on the decision
+    boundary P > u, where the kernel's fast estimate of P cannot decide and
+    the reference expression (cu:113-135) must.  Contacts cover the whole run,
+    so an exposure is a sum of c equal window weights and its probability is
+    one of a few values P_c, computed here in the reference's fp32 order.
+    Both streams must give the oracle's trajectory bit for bit, and the two
+    oracle trajectories must differ (the ties are hit)."""
+    import ctypes as C
+    libm = C.CDLL("libm.so.6")
+    libm.logf.argtypes = [C.c_float]
+    libm.logf.restype = C.c_float
+    rng = np.random.default_rng(11)
+    n, n_c = 64, 300
+    a = rng.integers(0, n, n_c)
+    b = (a + 1 + rng.integers(0, n - 1, n_c)) % n
+    t0 = 946713600
+    j = np.arange(n_c)
+    s = (t0 - 5000 - j).astype(np.int64)  # distinct, none nested in another
+    e = (t0 + 10 ** 6 - j).astype(np.int64)
+    beta = 0.3
+    p = Params(M=130, step_size=step, static_network_duration=D, iterations=40,
+               initial_infected=0.08, infect_prob=beta, exposed_duration=0,
+               infectious_duration=int(6 * step), resistant_duration=int(2 * step),
+               start_time=t0)
+    f32 = np.float32
+    log_base = f32(libm.logf(f32(1.0) - f32(beta)))
+    # window weights: D / steps_in_slice for every slice length that occurs
+    weights = {f32(D)} | {f32(f32(D) * (f32(1.0) / f32(k))) for k in range(1, 6)}
+    P = set()
+    for w in weights:
+        E = f32(0.0)
+        for c in range(1, 40):
+            E = f32(E + w)  # ascending-column fp32 sum of c equal weights
+            x = f32(log_base * f32(E / f32(3600.0)))
+            P.add(f32(f32(1.0) - f32(oracle.expf(float(x)))))
+    P = np.array(sorted(P), np.float32)
+    size = (p.iterations + 1) * p.M * n
+    pick = P[rng.integers(0, P.size, size)]
+    seed_u = rng.uniform(size=p.M * n).astype(np.float32)  # slot 0: u < initial_infected
+    g = oracle.OracleGraph.from_contacts(n, a.astype(np.int32), b.astype(np.int32), s, e)
+    results = []
+    for below in (False, True):
+        u = np.nextafter(pick, f32(0.0)) if below else pick.copy()
+        u[:p.M * n] = seed_u
+        counts, state, _ = g.simulate(_to_oracle_params(oracle, p, uniforms=u), want_state=True)
+        with Simulation.from_contacts(n, a.astype(np.int32), b.astype(np.int32), s, e, p) as sim:
+            sim.set_uniforms(u)
+            sim.run()
+            np.testing.assert_array_equal(sim.counts(), counts)
+            np.testing.assert_array_equal(sim.state(), state)
+        results.append(counts)
+    assert not np.array_equal(results[0], results[1])
+
+
 def test_truncated_windows_trajectory(oracle):
     """Windows whose query exceeds the hit cap (every contact overlapping)."""
     n = 6
