@@ -227,7 +227,7 @@ __device__ __forceinline__ v4i load_chunk(const int4 *entries, int e, int e1, in
 // SparseDenseProduct.h:64-70), visiting only entries that overlap the window
 // with a nonzero word: a column's entries share one neighbour, so a column is
 // skipped whole or not at all, and adding nothing leaves the fp32 sum as it is.
-template <bool kIntMode, int WT>
+template <bool kIntMode, int WT, bool kFull>
 __device__ __forceinline__ void group_exposure(const StepArgs &a, int e0, int e1,
                                                uint32_t rank_limit, int q0, int nq, int lane,
                                                const uint64_t (&ss)[WT], const v4i &en0,
@@ -254,10 +254,25 @@ __device__ __forceinline__ void group_exposure(const StepArgs &a, int e0, int e1
     ov = o;
     col = o > 0 ? en.x : 0;
   };
+  // kFull: W is a multiple of WT, so every group is whole and, WT being
+  // even, the row offset col * W + q0 is even and 16-byte aligned.  Such a
+  // group is read two words per load: a divergent gather costs the vector
+  // L1 one tag lookup per lane and load, so 16-byte loads halve the lookups
+  // of 8-byte ones for the same bytes.
   auto fetch = [&](int col, uint64_t (&raw)[WT]) {
     const uint64_t *mrow = a.I_cur + static_cast<size_t>(col) * a.W + q0;
+    if (kFull && WT % 2 == 0) {
+      const ulonglong2 *m2 = reinterpret_cast<const ulonglong2 *>(mrow);
 #pragma unroll
-    for (int q = 0; q < WT; ++q) raw[q] = mrow[q < nq ? q : nq - 1];
+      for (int h = 0; h < WT / 2; ++h) {
+        const ulonglong2 v = m2[h];
+        raw[2 * h] = v.x;
+        raw[2 * h + 1] = v.y;
+      }
+    } else {
+#pragma unroll
+      for (int q = 0; q < WT; ++q) raw[q] = mrow[q < nq ? q : nq - 1];
+    }
   };
   auto mask = [&](int ov, const uint64_t (&raw)[WT], uint32_t (&lo)[WT], uint32_t (&hi)[WT]) {
 #pragma unroll
@@ -423,7 +438,7 @@ __device__ __forceinline__ uint64_t draw_word(const StepArgs &a, const NetDev &n
 // Phase 1 of an iteration: one wavefront per node computes the exposure of
 // its susceptible replicas, draws, and writes the node's infection words
 // (zero words included) into ring slot k.
-template <bool kIntMode, bool kInjected, int WT>
+template <bool kIntMode, bool kInjected, int WT, bool kFull>
 __global__ __launch_bounds__(kBlock) void expo_kernel(StepArgs a) {
   const int lane = threadIdx.x & (kWave - 1);
   // Workgroups are dealt to the 8 XCDs round-robin; remap so that XCD x
@@ -452,7 +467,7 @@ __global__ __launch_bounds__(kBlock) void expo_kernel(StepArgs a) {
   const NetDev nd = a.nets[net];
   const uint32_t rlim = a.rank_limit ? a.rank_limit[net] : UINT32_MAX;
   for (int q0 = 0; q0 < a.W; q0 += WT) {
-    const int nq = min(WT, a.W - q0);
+    const int nq = kFull ? WT : min(WT, a.W - q0);
     // lane q holds susceptible word q0 + q (one load for the group)
     const uint64_t sv = q0 == 0 ? s_first : S[q0 + min(lane, nq - 1)];
     uint64_t ss[WT];
@@ -469,7 +484,7 @@ __global__ __launch_bounds__(kBlock) void expo_kernel(StepArgs a) {
     for (int q = 0; q < WT; ++q) x[q] = 0ull;
     if (any_sus != 0ull && (e1 > e0 || kInjected)) {
       float E[WT];
-      group_exposure<kIntMode, WT>(a, e0, e1, rlim, q0, nq, lane, ss, en0, en1, E);
+      group_exposure<kIntMode, WT, kFull>(a, e0, e1, rlim, q0, nq, lane, ss, en0, en1, E);
       uint32_t pw[4] = {0u, 0u, 0u, 0u};
       int pw_grp = -1;
 #pragma unroll
@@ -512,7 +527,7 @@ __global__ __launch_bounds__(kBlock) void long_row_kernel(StepArgs a, const int3
     float E[1];
     const v4i en0 = load_chunk(a.entries, e0, e1, lane);  // rows here hold > 2 chunks
     const v4i en1 = load_chunk(a.entries, e0 + kWave, e1, lane);
-    group_exposure<kIntMode, 1>(a, e0, e1, rlim, q, 1, lane, ss, en0, en1, E);
+    group_exposure<kIntMode, 1, false>(a, e0, e1, rlim, q, 1, lane, ss, en0, en1, E);
     uint32_t pw[4] = {0u, 0u, 0u, 0u};
     int pw_grp = -1;
     x = draw_word<kInjected>(a, nd, net, i, q, lane, ss[0], E[0], pw, pw_grp);
@@ -1098,12 +1113,21 @@ template <bool kInt, bool kInj>
 void launch_expo(const StepArgs &a, int variant, hipStream_t st) {
   const unsigned nb = (a.N + kWavesPerBlock - 1) / kWavesPerBlock;
   const dim3 grid((nb + kXcds - 1) / kXcds * kXcds);
-  if (variant == REPLAY_V_NODE16)
-    hipLaunchKernelGGL((expo_kernel<kInt, kInj, 16>), grid, dim3(kBlock), 0, st, a);
-  else if (variant == REPLAY_V_NODE4)
-    hipLaunchKernelGGL((expo_kernel<kInt, kInj, 4>), grid, dim3(kBlock), 0, st, a);
-  else
-    hipLaunchKernelGGL((expo_kernel<kInt, kInj, 1>), grid, dim3(kBlock), 0, st, a);
+  // whole groups (W a multiple of the group width) take the kernel whose
+  // neighbour-word loads are 16 bytes wide
+  if (variant == REPLAY_V_NODE16) {
+    if (a.W % 16 == 0)
+      hipLaunchKernelGGL((expo_kernel<kInt, kInj, 16, true>), grid, dim3(kBlock), 0, st, a);
+    else
+      hipLaunchKernelGGL((expo_kernel<kInt, kInj, 16, false>), grid, dim3(kBlock), 0, st, a);
+  } else if (variant == REPLAY_V_NODE4) {
+    if (a.W % 4 == 0)
+      hipLaunchKernelGGL((expo_kernel<kInt, kInj, 4, true>), grid, dim3(kBlock), 0, st, a);
+    else
+      hipLaunchKernelGGL((expo_kernel<kInt, kInj, 4, false>), grid, dim3(kBlock), 0, st, a);
+  } else {
+    hipLaunchKernelGGL((expo_kernel<kInt, kInj, 1, false>), grid, dim3(kBlock), 0, st, a);
+  }
 }
 
 // Planes the bit-sliced counters need for rows of up to maxrow contacts
