@@ -155,6 +155,29 @@ def test_trajectory_parity(oracle, case, seed):
         assert counts[:, :, 1].max() > 0
 
 
+@pytest.mark.parametrize("M", [1024, 1100])
+@pytest.mark.parametrize("step,D", [(600.0, 600), (1000.0, 3600)])
+def test_every_variant_matches_oracle_wide(oracle, M, step, D):
+    """Every exposure variant against the oracle at 16 and 18 words per node,
+    in integer (window = step) and float windows: W = 16 takes the
+    whole-group kernels of NODE4 and NODE16 (16-byte neighbour-word loads),
+    W = 18 their partial last group."""
+    n, a, b, s, e = _random_case(21, n=70, e=2000)
+    p = Params(M=M, step_size=step, static_network_duration=D, iterations=50,
+               initial_infected=0.1, infect_prob=0.3, exposed_duration=1200,
+               infectious_duration=5400, resistant_duration=3600, seed=17)
+    g = oracle.OracleGraph.from_contacts(n, a, b, s, e)
+    counts, state, _ = g.simulate(_to_oracle_params(oracle, p, threads=8), want_state=True)
+    assert counts[:, :, 1].max() > 0
+    with Simulation.from_contacts(n, a, b, s, e, p) as sim:
+        for v in (replay_amd.V_WORD, replay_amd.V_NODE4, replay_amd.V_NODE16, replay_amd.V_SLICED):
+            sim.set_variant(v)
+            sim.reset()
+            sim.run()
+            np.testing.assert_array_equal(sim.counts(), counts, err_msg=f"variant {v}")
+            np.testing.assert_array_equal(sim.state(), state, err_msg=f"variant {v}")
+
+
 def test_injected_uniform_stream(oracle):
     n, a, b, s, e = _random_case(3, n=50, e=1200)
     p = Params(M=70, step_size=600.0, static_network_duration=1800, iterations=80,
