@@ -29,9 +29,11 @@
 // the lanes load 64 of the node's packed contact entries at a time
 // (coalesced), gather the neighbours' infectious words and reduce them to
 // per-replica exposure -- whole-window contacts by a butterfly column count
-// across the lanes, partial overlaps one contact at a time -- then convert
-// exposure to 1 - exp(log(1-p) * E/3600), draw Philox uniforms and ballot the
-// infections into the ring slot of the iteration.  update_kernel then gives a
+// across the lanes, partial overlaps one contact at a time -- then draw
+// Philox uniforms, compare each with 1 - exp(log(1-p) * E/3600) (from a
+// hardware-exponential estimate, or the exact expression when the uniform
+// lies within 1e-5 of it) and ballot the infections into the ring slot of
+// the iteration.  update_kernel then gives a
 // thread to each word and applies the flips above.  Variants NODE4/NODE16
 // keep 4 or 16 of a node's words in registers per pass over its contacts
 // (WORD: one); SLICED gives each lane one (node, word) and counts contacts in
