@@ -109,6 +109,42 @@ int32_t rp_host_slice(rp_host_plan *h, int32_t k, int32_t *rowptr, int32_t *coli
   return 0;
 }
 
+// Packed entries of epoch e as {col, ts, te, rank} rows of 4 int64, rowptr
+// (N+1 entries, epoch-relative) into rowptr_out; returns the entry count, or
+// -1 when cap is too small.
+int64_t rp_host_plan_epoch(rp_host_plan *h, int32_t e, int32_t *rowptr_out, int64_t *out,
+                           int64_t cap) {
+  const int64_t N = h->net.n_nodes;
+  const int64_t b0 = h->slices.epoch_entry0[e], b1 = h->slices.epoch_entry0[e + 1];
+  if (b1 - b0 > cap) return -1;
+  std::copy(h->slices.rowptr.begin() + static_cast<size_t>(e) * (N + 1),
+            h->slices.rowptr.begin() + static_cast<size_t>(e + 1) * (N + 1), rowptr_out);
+  for (int64_t q = b0; q < b1; ++q) {
+    const rp::Entry &en = h->slices.entries[static_cast<size_t>(q)];
+    int64_t *o = out + 4 * (q - b0);
+    o[0] = en.col;
+    o[1] = en.ts;
+    o[2] = en.te;
+    o[3] = en.rank;
+  }
+  return b1 - b0;
+}
+
+// Long rows of epoch e (rows with more than the long-row threshold of
+// entries) and their integer-sum flags; returns their count.
+int64_t rp_host_plan_long_rows(rp_host_plan *h, int32_t e, int32_t *rows, uint8_t *int_ok,
+                               int64_t cap) {
+  const auto &sl = h->slices;
+  const int64_t q0 = sl.long_off[e], q1 = sl.long_off[e + 1];
+  for (int64_t q = q0; q < q1 && q - q0 < cap; ++q) {
+    rows[q - q0] = sl.long_row[static_cast<size_t>(q)];
+    int_ok[q - q0] = sl.long_int[static_cast<size_t>(q)];
+  }
+  return q1 - q0;
+}
+
+int32_t rp_host_long_row_threshold() { return rp::kLongRow; }
+
 void rp_host_plan_close(rp_host_plan *h) { delete h; }
 
 }  // extern "C"

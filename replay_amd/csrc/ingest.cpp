@@ -265,6 +265,66 @@ unsigned worker_count() {
   return std::min(n, 16u);
 }
 
+// Runs f(t, lo, hi) over contiguous chunks [lo, hi) of [0, n), one per
+// worker thread t; inline when n has fewer than two chunks of min_chunk.
+template <typename F>
+void par_chunks(int64_t n, int64_t min_chunk, F f) {
+  const int64_t T = std::max<int64_t>(
+      1, std::min<int64_t>(worker_count(), n / std::max<int64_t>(min_chunk, 1)));
+  if (T <= 1) {
+    f(0, int64_t(0), n);
+    return;
+  }
+  std::vector<std::thread> th;
+  for (int64_t t = 0; t < T; ++t)
+    th.emplace_back([&f, t, n, T] { f(static_cast<int>(t), n * t / T, n * (t + 1) / T); });
+  for (auto &x : th) x.join();
+}
+
+// Stable sort on the worker threads: chunks are stable-sorted, then merged
+// in pairs (std::merge takes the left run's element on ties), which yields
+// the one order a stable sort defines.
+template <typename T, typename C>
+void par_stable_sort(std::vector<T> &v, C cmp) {
+  const int64_t n = static_cast<int64_t>(v.size());
+  const int64_t parts = std::min<int64_t>(worker_count(), n / 16384);
+  if (parts <= 1) {
+    std::stable_sort(v.begin(), v.end(), cmp);
+    return;
+  }
+  std::vector<int64_t> bound(static_cast<size_t>(parts) + 1);
+  for (int64_t t = 0; t <= parts; ++t) bound[static_cast<size_t>(t)] = n * t / parts;
+  {
+    std::vector<std::thread> th;
+    for (int64_t t = 0; t < parts; ++t)
+      th.emplace_back([&, t] {
+        std::stable_sort(v.begin() + bound[static_cast<size_t>(t)],
+                         v.begin() + bound[static_cast<size_t>(t) + 1], cmp);
+      });
+    for (auto &x : th) x.join();
+  }
+  std::vector<T> buf(v.size());
+  while (bound.size() > 2) {
+    const size_t runs = bound.size() - 1;
+    std::vector<std::thread> th;
+    for (size_t r = 0; r < runs; r += 2) {
+      th.emplace_back([&, r] {
+        const auto a = v.begin() + bound[r], m = v.begin() + bound[r + 1];
+        if (r + 1 < runs)
+          std::merge(a, m, m, v.begin() + bound[r + 2], buf.begin() + bound[r], cmp);
+        else
+          std::copy(a, m, buf.begin() + bound[r]);
+      });
+    }
+    for (auto &x : th) x.join();
+    std::vector<int64_t> next;
+    for (size_t r = 0; r < runs; r += 2) next.push_back(bound[r]);
+    next.push_back(bound.back());
+    bound.swap(next);
+    v.swap(buf);
+  }
+}
+
 }  // namespace
 
 int parse_line(const char *line, int64_t len, int fast, int32_t *src, int32_t *tgt,
@@ -434,7 +494,7 @@ int finalize(Network &net, std::string &err) {
   }
   // Rank order: start ascending, end descending, file order on ties
   // (idb:30-46 under glibc 2.35's stable merge-sort qsort).
-  std::stable_sort(cs.begin(), cs.end(), [](const Contact &x, const Contact &y) {
+  par_stable_sort(cs, [](const Contact &x, const Contact &y) {
     if (x.start != y.start) return x.start < y.start;
     return x.end > y.end;
   });
@@ -535,7 +595,7 @@ int plan_windows(const Network &net, const replay_params &p, Plan &plan,
   const int64_t E = static_cast<int64_t>(cs.size());
   std::vector<int64_t> ends(static_cast<size_t>(E));
   for (int64_t k = 0; k < E; ++k) ends[k] = cs[k].end;
-  std::sort(ends.begin(), ends.end());
+  par_stable_sort(ends, [](int64_t x, int64_t y) { return x < y; });
   const int64_t cap = std::max<int64_t>(net.nlists, 1);
   for (auto &w : plan.windows) {
     w.scale = 1.0f / static_cast<float>(w.steps);
@@ -618,19 +678,27 @@ int build_slices(const Network &net, Plan &plan, int32_t windows_per_epoch,
   // Window span of every contact with positive length (zero-length contacts
   // count as hits but never carry weight).
   std::vector<Span> span(static_cast<size_t>(E));
-  for (int64_t k = 0; k < E; ++k) {
-    const Contact &c = cs[k];
-    Span s{1, 0};
-    if (c.end > c.start) {
-      s.a = static_cast<int32_t>(std::upper_bound(we.begin(), we.end(), c.start) - we.begin());
-      s.b = static_cast<int32_t>(std::lower_bound(ws.begin(), ws.end(), c.end) - ws.begin()) - 1;
+  par_chunks(E, 1 << 14, [&](int, int64_t lo, int64_t hi) {
+    for (int64_t k = lo; k < hi; ++k) {
+      const Contact &c = cs[k];
+      Span s{1, 0};
+      if (c.end > c.start) {
+        s.a = static_cast<int32_t>(std::upper_bound(we.begin(), we.end(), c.start) - we.begin());
+        s.b = static_cast<int32_t>(std::lower_bound(ws.begin(), ws.end(), c.end) - ws.begin()) - 1;
+      }
+      span[k] = s;
     }
-    span[k] = s;
-  }
+  });
   auto bytes_for = [&](int64_t S) {
+    std::vector<int64_t> part(worker_count(), 0);
+    par_chunks(E, 1 << 14, [&](int t, int64_t lo, int64_t hi) {
+      int64_t ne = 0;
+      for (int64_t k = lo; k < hi; ++k)
+        if (span[k].a <= span[k].b) ne += 2 * (span[k].b / S - span[k].a / S + 1);
+      part[static_cast<size_t>(t)] = ne;
+    });
     int64_t ne = 0;
-    for (int64_t k = 0; k < E; ++k)
-      if (span[k].a <= span[k].b) ne += 2 * (span[k].b / S - span[k].a / S + 1);
+    for (int64_t x : part) ne += x;
     int64_t ep = (W + S - 1) / S;
     return ne * static_cast<int64_t>(sizeof(Entry)) + ep * (N + 1) * 4;
   };
@@ -687,53 +755,106 @@ int build_slices(const Network &net, Plan &plan, int32_t windows_per_epoch,
     }
   }
   out.epoch_entry0[n_epochs] = total;
+  // Keys g = e * N + row split into contiguous ranges of about equal entry
+  // counts, one per worker.  Each worker walks every contact in rank order
+  // and writes only its own keys' entries, so a row's entries land in rank
+  // order exactly as in one sequential pass.
+  const int64_t K = static_cast<int64_t>(n_epochs) * N;
+  const int64_t parts = std::max<int64_t>(1, std::min<int64_t>(worker_count(), total / 65536));
+  std::vector<int64_t> kcut(static_cast<size_t>(parts) + 1, K);
+  kcut[0] = 0;
+  for (int64_t t = 1; t < parts; ++t)
+    kcut[static_cast<size_t>(t)] =
+        std::upper_bound(cnt.begin(), cnt.begin() + K, total * t / parts) - cnt.begin() - 1;
+  for (int64_t t = 1; t <= parts; ++t)
+    kcut[static_cast<size_t>(t)] = std::max(kcut[static_cast<size_t>(t)], kcut[static_cast<size_t>(t) - 1]);
   std::vector<int64_t> fill(cnt.begin(), cnt.end() - 1);
-  for (int64_t k = 0; k < E; ++k) {
-    const Span s = span[k];
-    if (s.a > s.b) continue;
-    const Contact &c = cs[k];
-    for (int64_t e = s.a / S; e <= s.b / S; ++e) {
-      const int64_t t0 = out.epoch_t0[e];
-      const int32_t klast = static_cast<int32_t>(std::min<int64_t>(W, (e + 1) * S)) - 1;
-      const int64_t t1 = we[klast];
-      Entry en;
-      en.ts = static_cast<int32_t>(std::max(c.start, t0) - t0);
-      en.te = static_cast<int32_t>(std::min(c.end, t1) - t0);
-      en.rank = static_cast<uint32_t>(k);
-      en.col = c.b;
-      out.entries[static_cast<size_t>(fill[static_cast<size_t>(e * N + c.a)]++)] = en;
-      en.col = c.a;
-      out.entries[static_cast<size_t>(fill[static_cast<size_t>(e * N + c.b)]++)] = en;
+  par_chunks(parts, 1, [&](int, int64_t t_lo, int64_t t_hi) {
+    for (int64_t t = t_lo; t < t_hi; ++t) {
+      const int64_t g_lo = kcut[static_cast<size_t>(t)], g_hi = kcut[static_cast<size_t>(t) + 1];
+      if (g_lo >= g_hi) continue;
+      for (int64_t k = 0; k < E; ++k) {
+        const Span s = span[k];
+        if (s.a > s.b) continue;
+        const Contact &c = cs[k];
+        for (int64_t e = s.a / S; e <= s.b / S; ++e) {
+          const int64_t ga = e * N + c.a, gb = e * N + c.b;
+          const bool in_a = ga >= g_lo && ga < g_hi, in_b = gb >= g_lo && gb < g_hi;
+          if (!in_a && !in_b) continue;
+          const int64_t t0 = out.epoch_t0[e];
+          const int32_t klast = static_cast<int32_t>(std::min<int64_t>(W, (e + 1) * S)) - 1;
+          const int64_t t1 = we[klast];
+          Entry en;
+          en.ts = static_cast<int32_t>(std::max(c.start, t0) - t0);
+          en.te = static_cast<int32_t>(std::min(c.end, t1) - t0);
+          en.rank = static_cast<uint32_t>(k);
+          if (in_a) {
+            en.col = c.b;
+            out.entries[static_cast<size_t>(fill[static_cast<size_t>(ga)]++)] = en;
+          }
+          if (in_b) {
+            en.col = c.a;
+            out.entries[static_cast<size_t>(fill[static_cast<size_t>(gb)]++)] = en;
+          }
+        }
+      }
     }
-  }
+  });
   // Columns ascending within a row, rank order kept among equal columns.
   // Integer sums are exact while a row's weights add up to less than 2^24:
   // the bound of short rows decides each window's mode, long rows carry
   // their own flag.
-  int64_t max_row = 0;
-  std::vector<int64_t> row_bound(static_cast<size_t>(n_epochs), 0);
+  // Rows sorted and bounded on the same key ranges; each worker keeps its
+  // own maxima and long-row list, merged afterwards in key order.
   const int64_t D = we[0] - ws[0];
-  out.long_off.assign(static_cast<size_t>(n_epochs) + 1, 0);
-  for (int32_t e = 0; e < n_epochs; ++e) {
-    const int32_t *rp = &out.rowptr[static_cast<size_t>(e) * (N + 1)];
-    Entry *base = out.entries.data() + out.epoch_entry0[e];
-    for (int64_t r = 0; r < N; ++r) {
-      Entry *b = base + rp[r], *x = base + rp[r + 1];
-      int64_t len = x - b;
-      if (len > 1)
-        std::stable_sort(b, x, [](const Entry &u, const Entry &v) { return u.col < v.col; });
-      int64_t bound = 0;
-      for (Entry *q = b; q < x; ++q) bound += std::min<int64_t>(q->te - q->ts, D);
-      if (len > kLongRow) {
-        out.long_row.push_back(static_cast<int32_t>(r));
-        out.long_int.push_back(bound < (int64_t(1) << 24) ? 1 : 0);
-      } else {
-        max_row = std::max(max_row, len);
-        row_bound[e] = std::max(row_bound[e], bound);
+  struct RowStats {
+    int64_t max_row = 0;
+    std::vector<int64_t> row_bound;
+    std::vector<int64_t> long_key;
+    std::vector<uint8_t> long_int;
+  };
+  std::vector<RowStats> stats(static_cast<size_t>(parts));
+  par_chunks(parts, 1, [&](int, int64_t t_lo, int64_t t_hi) {
+    for (int64_t t = t_lo; t < t_hi; ++t) {
+      RowStats &st = stats[static_cast<size_t>(t)];
+      st.row_bound.assign(static_cast<size_t>(n_epochs), 0);
+      for (int64_t g = kcut[static_cast<size_t>(t)]; g < kcut[static_cast<size_t>(t) + 1]; ++g) {
+        const int64_t e = g / N, r = g - e * N;
+        const int32_t *rp = &out.rowptr[static_cast<size_t>(e) * (N + 1)];
+        Entry *base = out.entries.data() + out.epoch_entry0[e];
+        Entry *b = base + rp[r], *x = base + rp[r + 1];
+        const int64_t len = x - b;
+        if (len > 1)
+          std::stable_sort(b, x, [](const Entry &u, const Entry &v) { return u.col < v.col; });
+        int64_t bound = 0;
+        for (Entry *q = b; q < x; ++q) bound += std::min<int64_t>(q->te - q->ts, D);
+        if (len > kLongRow) {
+          st.long_key.push_back(g);
+          st.long_int.push_back(bound < (int64_t(1) << 24) ? 1 : 0);
+        } else {
+          st.max_row = std::max(st.max_row, len);
+          st.row_bound[static_cast<size_t>(e)] = std::max(st.row_bound[static_cast<size_t>(e)], bound);
+        }
       }
     }
-    out.long_off[static_cast<size_t>(e) + 1] = static_cast<int64_t>(out.long_row.size());
+  });
+  int64_t max_row = 0;
+  std::vector<int64_t> row_bound(static_cast<size_t>(n_epochs), 0);
+  out.long_off.assign(static_cast<size_t>(n_epochs) + 1, 0);
+  for (const RowStats &st : stats) {
+    max_row = std::max(max_row, st.max_row);
+    for (int32_t e = 0; e < n_epochs; ++e)
+      row_bound[static_cast<size_t>(e)] =
+          std::max(row_bound[static_cast<size_t>(e)], st.row_bound[static_cast<size_t>(e)]);
+    for (size_t q = 0; q < st.long_key.size(); ++q) {
+      const int64_t e = st.long_key[q] / N;
+      out.long_row.push_back(static_cast<int32_t>(st.long_key[q] - e * N));
+      out.long_int.push_back(st.long_int[q]);
+      out.long_off[static_cast<size_t>(e) + 1]++;
+    }
   }
+  for (int32_t e = 0; e < n_epochs; ++e)
+    out.long_off[static_cast<size_t>(e) + 1] += out.long_off[static_cast<size_t>(e)];
   out.max_row_entries = max_row;
   for (auto &w : plan.windows)
     w.int_mode = (w.scale == 1.0f && row_bound[w.epoch] < (int64_t(1) << 24)) ? 1 : 0;

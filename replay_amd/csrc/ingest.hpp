@@ -51,11 +51,15 @@ struct Plan {
 
 // One packed entry: contact seen from `row`, neighbour `col`, its interval
 // clipped to the epoch and made relative to the epoch base time, and its rank.
+// The empty constructor leaves a new entry unwritten, so sizing an array of
+// hundreds of millions of them does not first zero it: every entry is
+// written by the packing pass.
 struct Entry {
   int32_t col;
   int32_t ts;
   int32_t te;
   uint32_t rank;
+  Entry() {}
 };
 
 // Rows with more entries than this in an epoch are "long": the step kernels

@@ -1466,17 +1466,35 @@ int finish_open(replay_ctx *ctx) {
   }
   ctx->epoch_entry0[n_ep] = total;
   ctx->n_entries = total;
-  std::vector<rp::Entry> entries(static_cast<size_t>(total));
-  for (int32_t e = 0; e < n_ep; ++e) {
-    int64_t dst = ctx->epoch_entry0[e];
-    for (auto &nh : ctx->nets) {
-      const int64_t b0 = nh.slices.epoch_entry0[e], b1 = nh.slices.epoch_entry0[e + 1];
-      for (int64_t q = b0; q < b1; ++q) {
-        rp::Entry en = nh.slices.entries[static_cast<size_t>(q)];
-        en.col += nh.node_off;
-        entries[static_cast<size_t>(dst++)] = en;
+  // Epoch e holds the networks' epoch-e entries one network after another,
+  // columns shifted by the network's first node.  One network: its array is
+  // already that layout (shift 0) and is taken over; several: each network
+  // copies its ranges on its own thread.
+  std::vector<rp::Entry> entries;
+  if (ctx->nets.size() == 1) {
+    entries.swap(ctx->nets[0].slices.entries);
+  } else {
+    entries.resize(static_cast<size_t>(total));
+    std::vector<int64_t> dst0(static_cast<size_t>(n_ep) * ctx->nets.size());
+    for (int32_t e = 0; e < n_ep; ++e) {
+      int64_t dst = ctx->epoch_entry0[e];
+      for (size_t n = 0; n < ctx->nets.size(); ++n) {
+        dst0[static_cast<size_t>(e) * ctx->nets.size() + n] = dst;
+        const auto &sl = ctx->nets[n].slices;
+        dst += sl.epoch_entry0[e + 1] - sl.epoch_entry0[e];
       }
     }
+    parallel_for(ctx->nets.size(), [&](size_t n) {
+      const NetHost &nh = ctx->nets[n];
+      for (int32_t e = 0; e < n_ep; ++e) {
+        int64_t dst = dst0[static_cast<size_t>(e) * ctx->nets.size() + n];
+        for (int64_t q = nh.slices.epoch_entry0[e]; q < nh.slices.epoch_entry0[e + 1]; ++q) {
+          rp::Entry en = nh.slices.entries[static_cast<size_t>(q)];
+          en.col += nh.node_off;
+          entries[static_cast<size_t>(dst++)] = en;
+        }
+      }
+    });
   }
   for (auto &nh : ctx->nets) {  // per-network copies are no longer needed
     std::vector<rp::Entry>().swap(nh.slices.entries);
