@@ -461,6 +461,12 @@ __global__ __launch_bounds__(kBlock) void expo_kernel(StepArgs a) {
   const int net = a.net_of[i];
   const uint64_t s_first = S[min(lane, min(WT, a.W) - 1)];
   if (e1 - e0 > a.long_row) return;  // long_row_kernel takes it
+  if (!kInjected && e1 <= e0) {
+    // no contact in the epoch: no exposure, so no infection (a draw needs
+    // E > 0); the node's words of ring slot k are zero
+    for (int w = lane; w < a.W; w += kWave) X_k[w] = 0ull;
+    return;
+  }
   v4i en0 = v4i{0, 0, 0, 0}, en1 = v4i{0, 0, 0, 0};
   if (e1 > e0) {
     en0 = load_chunk(a.entries, e0, e1, lane);
