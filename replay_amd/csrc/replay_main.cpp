@@ -15,11 +15,14 @@
 //     %d iterations..." (cu:835-836).
 // Differences: --cpu-only and --cpu-threads are accepted for compatibility
 // but the HIP engine always runs; --seed <u64> and --device <n> are added.
+#include <condition_variable>
 #include <cstdint>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "replay_hip.h"
@@ -270,15 +273,76 @@ int main(int argc, char **argv) {
   std::fflush(stdout);
 
   if (node_fp) {
-    // per-iteration state is needed: run one iteration at a time
+    // Per-iteration state is needed, so iterations run one at a time.  Each
+    // state goes to a writer thread through two host buffers: iteration k+1
+    // runs on the device while iteration k's rows are formatted.  The writer
+    // classifies every (replica, node) once, for the node row and for the
+    // replica's summary counts (cu:505-568 write both from the same state),
+    // so no count block is fetched per iteration.
+    std::vector<long long> times(static_cast<size_t>(T));
+    for (int32_t k = 0; k < T; ++k) times[static_cast<size_t>(k)] = replay_iteration_time(ctx, k);
+    std::vector<int32_t> buf[2];
+    buf[0].resize(static_cast<size_t>(N) * M);
+    buf[1].resize(static_cast<size_t>(N) * M);
+    std::mutex mu;
+    std::condition_variable cv;
+    int32_t queued = -1, written = -1;  // last iteration handed over / written
+    bool stop = false;
+    std::thread writer([&] {
+      for (int32_t k = 0;; ++k) {
+        {
+          std::unique_lock<std::mutex> lk(mu);
+          cv.wait(lk, [&] { return queued >= k || stop; });
+          if (queued < k) return;
+        }
+        const int32_t *st = buf[k & 1].data();
+        int32_t *c4 = counts.data() + static_cast<size_t>(k + 1) * M * 4;
+        for (int32_t sim = 0; sim < M; ++sim) {
+          int32_t n4[4] = {0, 0, 0, 0};  // susceptible, incubating, infectious, resistant
+          for (int32_t node = 0; node < N; ++node) {
+            const int cls = classify(st[static_cast<size_t>(sim) * N + node], upper,
+                                     p.exposed_duration, p.infectious_duration,
+                                     p.resistant_duration);
+            ++n4[cls];
+            node_out.row_node(times[static_cast<size_t>(k)], sim, node, kStateNames[cls]);
+          }
+          int32_t *c = c4 + static_cast<size_t>(sim) * 4;
+          c[0] = n4[1];
+          c[1] = n4[2];
+          c[2] = n4[3];
+          c[3] = n4[0];
+          if (summary_fp) sum_out.row_summary(times[static_cast<size_t>(k)], sim, c);
+        }
+        if (summary_fp) sum_out.flush();
+        node_out.flush();
+        {
+          std::lock_guard<std::mutex> lk(mu);
+          written = k;
+        }
+        cv.notify_all();
+      }
+    });
     for (int32_t k = 0; k < T && rc == REPLAY_OK; ++k) {
       rc = replay_run(ctx, 1);
-      if (rc == REPLAY_OK) rc = replay_get_state(ctx, state.data(), static_cast<int64_t>(state.size()));
-      if (rc == REPLAY_OK) rc = replay_get_counts(ctx, counts.data(), static_cast<int64_t>(k + 2) * M * 4);
+      {
+        // buffer k & 1 is free once iteration k - 2 is written
+        std::unique_lock<std::mutex> lk(mu);
+        cv.wait(lk, [&] { return written >= k - 2; });
+      }
       if (rc == REPLAY_OK)
-        write_block(replay_iteration_time(ctx, k), counts.data() + static_cast<size_t>(k + 1) * M * 4,
-                    state.data());
+        rc = replay_get_state(ctx, buf[k & 1].data(), static_cast<int64_t>(buf[k & 1].size()));
+      if (rc == REPLAY_OK) {
+        std::lock_guard<std::mutex> lk(mu);
+        queued = k;
+      }
+      cv.notify_all();
     }
+    {
+      std::lock_guard<std::mutex> lk(mu);
+      stop = true;
+    }
+    cv.notify_all();
+    writer.join();
   } else {
     rc = replay_run(ctx, T);
     if (rc == REPLAY_OK) rc = replay_get_counts(ctx, counts.data(), static_cast<int64_t>(counts.size()));

@@ -320,16 +320,20 @@ def _run(cmd):
     return r
 
 
-def test_cli_csv_byte_identical(oracle, tmp_path):
+@pytest.mark.parametrize("M,T", [(7, 40), (64, 400)])
+def test_cli_csv_byte_identical(oracle, tmp_path, M, T):
+    """Both CLI output paths against the oracle CLI, byte for byte: per
+    iteration (node states and summaries written by a writer thread from
+    double-buffered states) and all iterations at once (summaries only)."""
     path = graph_path("random_nested.txt")
-    args = [path, "--M", "7", "--step-size", "1200", "--iterations", "40", "--time-step", "3600",
+    args = [path, "--M", str(M), "--step-size", "1200", "--iterations", str(T), "--time-step", "3600",
             "--initial-infected", "0.2", "--infect-prob", "0.4", "--exposed-duration", "3600",
             "--infectious-duration", "7200", "--resistant-duration", "3600",
             "--start-time", str(_graph_start(path)), "--seed", "11"]
     ours = [str(tmp_path / "s1.csv"), str(tmp_path / "n1.csv")]
     ref = [str(tmp_path / "s2.csv"), str(tmp_path / "n2.csv")]
     r = _run([replay_amd.CLI_PATH, *args, "--summary-out", ours[0], "--node-state-out", ours[1]])
-    assert "Starting temporal Monte Carlo simulation for 40 iterations..." in r.stdout
+    assert f"Starting temporal Monte Carlo simulation for {T} iterations..." in r.stdout
     _run([os.path.join(ROOT, "oracle", "replay_cpu"), *args, "--summary-out", ref[0],
           "--node-state-out", ref[1]])
     for x, y in zip(ours, ref):
