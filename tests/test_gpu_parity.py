@@ -341,3 +341,23 @@ def test_cli_csv_byte_identical(oracle, tmp_path, M, T):
     # summary only (the all-iterations-at-once path)
     _run([replay_amd.CLI_PATH, *args, "-q", "--summary-out", ours[0]])
     assert open(ours[0]).read() == open(ref[0]).read()
+
+
+def test_summary_frame_matches_csv_writer(tmp_path):
+    """summary_frame over a simulation's counts and block times holds the
+    rows write_summary_csv writes (the reference CSV layout)."""
+    from replay_amd import summary, write_summary_csv
+    n, a, b, s, e = _random_case(5, n=40, e=800)
+    p = Params(M=5, step_size=900.0, static_network_duration=1800, iterations=25,
+               initial_infected=0.1, infect_prob=0.3, exposed_duration=1800,
+               infectious_duration=5400, resistant_duration=3600, start_time=946713600)
+    with Simulation.from_contacts(n, a, b, s, e, p) as sim:
+        sim.run()
+        f = tmp_path / "s.csv"
+        write_summary_csv(str(f), sim)
+        times, counts = summary.block_times(sim), sim.counts()
+    t2, c2 = summary.read_summary_csv(str(f))
+    np.testing.assert_array_equal(t2, times)
+    np.testing.assert_array_equal(c2, counts)
+    df = summary.summary_frame(counts, times)
+    assert df.shape == ((p.iterations + 1) * p.M, 6)
