@@ -256,6 +256,26 @@ int32_t replay_marginal_backward(replay_marginal *m, const float *grad_curves,
 
 void replay_marginal_destroy(replay_marginal *m);
 
+/* ---- Benchmark input generator ----
+   Replaces the group-structured generator executable
+   (generate_networks/graph_generator.c:39-157, main.c:39-65).  Writes to
+   path the file that generator writes for the same arguments on a glibc
+   system: nodes 1..population in round-robin groups (i % groups), one draw
+   per node pair and step from the C library's rand() stream after
+   srand(seed), a contact when draw / RAND_MAX < prob_inner (same group) or
+   prob_inter, two start/end draws base_time + step * step_size +
+   rand() % step_size in ascending order, lines "u, v, ISO, ISO;" under a
+   NODE_LIST header.  base_time is UTC seconds (the reference's main() reads
+   its start argument as local time).  n_contacts (may be NULL) receives the
+   number of contact lines; verbose != 0 prints the reference's progress
+   lines.  REPLAY_E_ARG for a negative seed, steps or population, or a
+   non-positive step size or group count; REPLAY_E_IO when path cannot be
+   written.  Needs no device. */
+int32_t replay_generate_groups(int64_t base_time, int32_t steps, int32_t step_size,
+                               int32_t population, int32_t groups, double prob_inner,
+                               double prob_inter, int32_t seed, const char *path,
+                               int64_t *n_contacts, int32_t verbose);
+
 /* Message of the last failed call on ctx (or of the last failed open on
    this thread when ctx is NULL). */
 const char *replay_last_error(replay_ctx *ctx);

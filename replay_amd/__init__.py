@@ -20,6 +20,7 @@ import numpy as np
 PKG_DIR = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(PKG_DIR, "lib", "libreplay_hip.so")
 CLI_PATH = os.path.join(PKG_DIR, "bin", "replay")
+GEN_PATH = os.path.join(PKG_DIR, "bin", "replay_gen")
 
 ABI_VERSION = 1
 
@@ -79,7 +80,7 @@ EXPORTED_SYMBOLS = (
     "replay_get_exposure", "replay_last_error", "replay_error_string", "replay_close",
     "replay_open_batch", "replay_get_network_offsets", "replay_autotune", "replay_set_variant",
     "replay_marginal_create", "replay_marginal_forward", "replay_marginal_backward",
-    "replay_marginal_destroy",
+    "replay_marginal_destroy", "replay_generate_groups",
 )
 
 _lib = None
@@ -147,6 +148,10 @@ def load_library() -> C.CDLL:
     L.replay_marginal_backward.argtypes = [vp, vp, vp, vp, vp, vp, vp]
     L.replay_marginal_destroy.argtypes = [vp]
     L.replay_marginal_destroy.restype = None
+    L.replay_generate_groups.argtypes = [C.c_int64, C.c_int32, C.c_int32, C.c_int32, C.c_int32,
+                                         C.c_double, C.c_double, C.c_int32, C.c_char_p,
+                                         P(C.c_int64), C.c_int32]
+    L.replay_generate_groups.restype = C.c_int32
     for name in ("replay_marginal_create", "replay_marginal_forward", "replay_marginal_backward"):
         getattr(L, name).restype = C.c_int32
     for name in ("replay_open", "replay_open_contacts", "replay_get_info", "replay_reset",
@@ -396,5 +401,23 @@ def write_summary_csv(path: str, sim: Simulation) -> None:
                 f.write(f"{t},{s},{c[0]},{c[1]},{c[2]},{c[3]}\n")
 
 
+def generate_groups(path: str, base_time: int, steps: int, step_size: int, population: int,
+                    groups: int, prob_inner: float, prob_inter: float, seed: int = 0,
+                    verbose: bool = False) -> int:
+    """Writes the group-structured contact graph of
+    generate_networks/graph_generator.c:39-157 for these arguments (base_time
+    in UTC seconds) to path; returns the number of contact lines.  Needs no
+    GPU."""
+    L = load_library()
+    n = C.c_int64(0)
+    rc = L.replay_generate_groups(int(base_time), int(steps), int(step_size), int(population),
+                                  int(groups), float(prob_inner), float(prob_inter), int(seed),
+                                  path.encode(), C.byref(n), 1 if verbose else 0)
+    if rc != 0:
+        raise ReplayError(rc, L.replay_error_string(rc).decode())
+    return n.value
+
+
 __all__ = ["Params", "Network", "Simulation", "ReplayError", "load_library", "library_path",
-           "write_summary_csv", "EXPORTED_SYMBOLS", "CLI_PATH", "LIB_PATH"]
+           "write_summary_csv", "generate_groups", "EXPORTED_SYMBOLS", "CLI_PATH", "GEN_PATH",
+           "LIB_PATH"]

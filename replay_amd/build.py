@@ -1,4 +1,5 @@
-"""Builds libreplay_hip.so (HIP kernels for gfx950 + C ABI) and the `replay` CLI.
+"""Builds libreplay_hip.so (HIP kernels for gfx950 + C ABI), the `replay` CLI
+and the `replay_gen` input generator.
 
     python -m replay_amd.build        # or replay_amd.build.build()
 
@@ -21,6 +22,7 @@ LIB_DIR = os.path.join(PKG, "lib")
 BIN_DIR = os.path.join(PKG, "bin")
 LIB = os.path.join(LIB_DIR, "libreplay_hip.so")
 CLI = os.path.join(BIN_DIR, "replay")
+GEN_CLI = os.path.join(BIN_DIR, "replay_gen")
 
 ARCH = os.environ.get("REPLAY_HIP_ARCH", "gfx950")
 HIPCC = os.environ.get("HIPCC", shutil.which("hipcc") or "/opt/rocm/bin/hipcc")
@@ -49,7 +51,7 @@ def build(quiet: bool = True) -> str:
     os.makedirs(LIB_DIR, exist_ok=True)
     os.makedirs(BIN_DIR, exist_ok=True)
     host_objs = []
-    for src in ("ingest.cpp", "host_slices.cpp"):
+    for src in ("ingest.cpp", "host_slices.cpp", "generator.cpp"):
         obj = os.path.join(BUILD, src.replace(".cpp", ".o"))
         _run(["g++", *CXX_FLAGS, "-c", os.path.join(CSRC, src), "-o", obj], quiet)
         host_objs.append(obj)
@@ -64,6 +66,8 @@ def build(quiet: bool = True) -> str:
     os.replace(tmp, LIB)
     _run(["g++", "-O2", "-std=c++17", f"-I{INCLUDE}", os.path.join(CSRC, "replay_main.cpp"),
           f"-L{LIB_DIR}", "-lreplay_hip", "-Wl,-rpath,$ORIGIN/../lib", "-o", CLI], quiet)
+    _run(["g++", "-O2", "-std=c++17", f"-I{INCLUDE}", os.path.join(CSRC, "replay_gen_main.cpp"),
+          f"-L{LIB_DIR}", "-lreplay_hip", "-Wl,-rpath,$ORIGIN/../lib", "-o", GEN_CLI], quiet)
     return LIB
 
 
