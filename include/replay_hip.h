@@ -276,6 +276,24 @@ int32_t replay_generate_groups(int64_t base_time, int32_t steps, int32_t step_si
                                double prob_inter, int32_t seed, const char *path,
                                int64_t *n_contacts, int32_t verbose);
 
+/* Replaces the command-file generator contactgraphgen/contactgraphgen.py
+   (process_command_file py:316-362, write_contact_graph py:365-381, main
+   py:395-418).  Reads the ADDNODE / ADDEDGE commands in commands_path and
+   writes to out_path the graph that script writes: NODE_LIST of the added
+   ids in ascending order, then every pair u < v of each ADDEDGE clique at
+   each start START + k DELTA (k < N for OCCURRENCES N, start < T for
+   UNTIL T, one start when DELTA is 0 or no clause), duplicates removed, in
+   (start, u, v, end) order, lines "u, v, ISO, ISO; ".  Rejected lines
+   print the script's messages on stderr (unless quiet) and are skipped.
+   n_contacts (may be NULL) receives the number of contact lines written.
+   REPLAY_E_IO when the command file cannot be read or the graph cannot be
+   written (nothing written), REPLAY_E_FORMAT for a command file that is not
+   UTF-8, REPLAY_E_OVERFLOW when a time passes 9999-12-31T23:59:59Z (the
+   file then ends before that line, as the script's does), REPLAY_E_NOMEM
+   beyond 2^30 contact occurrences.  Needs no device. */
+int32_t replay_contactgraph(const char *commands_path, const char *out_path,
+                            int64_t *n_contacts, int32_t quiet);
+
 /* Message of the last failed call on ctx (or of the last failed open on
    this thread when ctx is NULL). */
 const char *replay_last_error(replay_ctx *ctx);

@@ -21,6 +21,7 @@ PKG_DIR = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(PKG_DIR, "lib", "libreplay_hip.so")
 CLI_PATH = os.path.join(PKG_DIR, "bin", "replay")
 GEN_PATH = os.path.join(PKG_DIR, "bin", "replay_gen")
+CGG_PATH = os.path.join(PKG_DIR, "bin", "replay_cgg")
 
 ABI_VERSION = 1
 
@@ -80,7 +81,7 @@ EXPORTED_SYMBOLS = (
     "replay_get_exposure", "replay_last_error", "replay_error_string", "replay_close",
     "replay_open_batch", "replay_get_network_offsets", "replay_autotune", "replay_set_variant",
     "replay_marginal_create", "replay_marginal_forward", "replay_marginal_backward",
-    "replay_marginal_destroy", "replay_generate_groups",
+    "replay_marginal_destroy", "replay_generate_groups", "replay_contactgraph",
 )
 
 _lib = None
@@ -152,6 +153,8 @@ def load_library() -> C.CDLL:
                                          C.c_double, C.c_double, C.c_int32, C.c_char_p,
                                          P(C.c_int64), C.c_int32]
     L.replay_generate_groups.restype = C.c_int32
+    L.replay_contactgraph.argtypes = [C.c_char_p, C.c_char_p, P(C.c_int64), C.c_int32]
+    L.replay_contactgraph.restype = C.c_int32
     for name in ("replay_marginal_create", "replay_marginal_forward", "replay_marginal_backward"):
         getattr(L, name).restype = C.c_int32
     for name in ("replay_open", "replay_open_contacts", "replay_get_info", "replay_reset",
@@ -418,6 +421,27 @@ def generate_groups(path: str, base_time: int, steps: int, step_size: int, popul
     return n.value
 
 
+def contact_graph(commands_path: str, out_path: str, quiet: bool = False) -> int:
+    """Writes the contact graph of an ADDNODE / ADDEDGE command file, as
+    contactgraphgen/contactgraphgen.py does, and returns the number of
+    contact lines.  Messages for rejected command lines go to stderr unless
+    quiet.  Raises ReplayError when the command file cannot be read, the
+    graph cannot be written or a time passes 9999-12-31T23:59:59Z.  Needs no
+    GPU."""
+    L = load_library()
+    n = C.c_int64(0)
+    rc = L.replay_contactgraph(commands_path.encode(), out_path.encode(), C.byref(n),
+                               1 if quiet else 0)
+    if rc != 0:
+        what = {-2: "the command file cannot be read or the graph cannot be written",
+                -3: "the command file is not UTF-8",
+                -9: "the commands expand to more than 2^30 contact occurrences",
+                -11: f"a time passes 9999-12-31T23:59:59Z; {out_path} ends before it "
+                     f"({n.value} contact lines)"}
+        raise ReplayError(rc, what.get(rc, L.replay_error_string(rc).decode()))
+    return n.value
+
+
 __all__ = ["Params", "Network", "Simulation", "ReplayError", "load_library", "library_path",
-           "write_summary_csv", "generate_groups", "EXPORTED_SYMBOLS", "CLI_PATH", "GEN_PATH",
-           "LIB_PATH"]
+           "write_summary_csv", "generate_groups", "contact_graph", "EXPORTED_SYMBOLS", "CLI_PATH",
+           "GEN_PATH", "CGG_PATH", "LIB_PATH"]

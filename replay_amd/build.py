@@ -23,6 +23,7 @@ BIN_DIR = os.path.join(PKG, "bin")
 LIB = os.path.join(LIB_DIR, "libreplay_hip.so")
 CLI = os.path.join(BIN_DIR, "replay")
 GEN_CLI = os.path.join(BIN_DIR, "replay_gen")
+CGG_CLI = os.path.join(BIN_DIR, "replay_cgg")
 
 ARCH = os.environ.get("REPLAY_HIP_ARCH", "gfx950")
 HIPCC = os.environ.get("HIPCC", shutil.which("hipcc") or "/opt/rocm/bin/hipcc")
@@ -51,7 +52,7 @@ def build(quiet: bool = True) -> str:
     os.makedirs(LIB_DIR, exist_ok=True)
     os.makedirs(BIN_DIR, exist_ok=True)
     host_objs = []
-    for src in ("ingest.cpp", "host_slices.cpp", "generator.cpp"):
+    for src in ("ingest.cpp", "host_slices.cpp", "generator.cpp", "contactgraph.cpp"):
         obj = os.path.join(BUILD, src.replace(".cpp", ".o"))
         _run(["g++", *CXX_FLAGS, "-c", os.path.join(CSRC, src), "-o", obj], quiet)
         host_objs.append(obj)
@@ -68,6 +69,8 @@ def build(quiet: bool = True) -> str:
           f"-L{LIB_DIR}", "-lreplay_hip", "-Wl,-rpath,$ORIGIN/../lib", "-o", CLI], quiet)
     _run(["g++", "-O2", "-std=c++17", f"-I{INCLUDE}", os.path.join(CSRC, "replay_gen_main.cpp"),
           f"-L{LIB_DIR}", "-lreplay_hip", "-Wl,-rpath,$ORIGIN/../lib", "-o", GEN_CLI], quiet)
+    _run(["g++", "-O2", "-std=c++17", f"-I{INCLUDE}", os.path.join(CSRC, "replay_cgg_main.cpp"),
+          f"-L{LIB_DIR}", "-lreplay_hip", "-Wl,-rpath,$ORIGIN/../lib", "-o", CGG_CLI], quiet)
     return LIB
 
 
