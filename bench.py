@@ -18,8 +18,10 @@ config1/config2: one network, autotuned variant.  config4: the
 differentiable marginal form, forward + adjoint per calibration step.
 
 Prints one JSON line with a roofline object (algorithmic bytes per
-iteration over the HIP-event time per iteration; measured HBM traffic where
-a committed rocprofv3 summary holds it) and a CPU baseline: the oracle (a C
+iteration -- the window SpMM's bytes in fused form, from the window's
+nonzeros counted on the device -- over the HIP-event time per iteration;
+measured HBM traffic where a committed rocprofv3 summary holds it) and a
+CPU baseline: the oracle (a C
 restatement of the reference --cpu-only loop) timed on a bounded sample of
 the same workload on the host cores.
 """
@@ -35,6 +37,9 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, published
+# MI355X vector fp32 adds without packing: 256 CUs x 4 SIMDs x 16 lanes x
+# 2.4 GHz = 39.3 T lane-adds/s (the published 157.3 TFLOP/s counts packed FMAs)
+VALU_LANE_ADDS_T = 39.3
 
 
 def log(msg: str) -> None:
@@ -76,8 +81,29 @@ def measured_traffic(workload: str):
         return json.load(f)
 
 
-def roofline(info: dict, per_launch_s: float, workload: str) -> dict:
-    b = float(info["step_bytes"])
+def mean_nnz(sim, T: int):
+    """Mean nonzeros of the window CSR over the run's iterations (every
+    iteration, or 256 evenly spaced ones when there are more), counted on the
+    device; and which iterations were counted."""
+    if T <= 256:
+        ks = list(range(T))
+    else:
+        ks = sorted({round(i * (T - 1) / 255) for i in range(256)})
+    nnz = sum(sim.slice_nnz(k) for k in ks) / len(ks)
+    return nnz, ("every iteration" if T <= 256 else f"{len(ks)} evenly spaced iterations")
+
+
+def roofline(info: dict, per_launch_s: float, workload: str, nnz: float, nnz_note: str) -> dict:
+    """Algorithmic bytes of one iteration by the SpMM definition in its fused
+    form (the N x M exposure matrix is never written): 8 nnz for the window's
+    CSR + 4 (N + 1) row pointers + N M / 8 for the infectious-mask read + the
+    state read and written.  The state term is step_bytes less its entry
+    term; step_bytes (what this design streams, the slice epoch's packed
+    entries at entry_bytes each) is reported beside it."""
+    design = float(info["step_bytes"])
+    ent = info["n_entries"] / max(info["n_epochs"], 1)
+    state = design - int(info["entry_bytes"] * ent)
+    b = 8.0 * nnz + state
     achieved = b / per_launch_s / 1e9
     tr = measured_traffic(workload)
     return {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -86,10 +112,25 @@ def roofline(info: dict, per_launch_s: float, workload: str) -> dict:
             "traffic_source": tr["source"] if tr else None,
             "kernel": "step (expo_kernel + update_kernel)",
             "bytes_per_launch": b, "us_per_launch": per_launch_s * 1e6,
-            "bytes_definition": "per 64-replica word: susceptible word read+write, own "
-                                "infectious word read, next infectious word write, infection "
-                                "word write, one ring read per timer offset; plus the slice "
-                                "epoch's 16-byte contact entries and row pointers"}
+            "bytes_definition": "SpMM bytes of the window in fused form: 8 nnz (window CSR "
+                                f"nonzeros, mean over {nnz_note}, counted on the device) + "
+                                "4 (N+1) row pointers + N*M/8 infectious-mask read + state "
+                                "read and written per 64-replica word (susceptible read+write, "
+                                "next infectious write, infection word write, one ring read "
+                                "per timer offset) + per-replica counts",
+            "nnz_per_iteration": nnz,
+            "design_bytes_per_launch": design,
+            "design_bytes_definition": "the same state terms plus the slice epoch's packed "
+                                       f"contact entries at {info['entry_bytes']} bytes each, "
+                                       "which the exposure kernel streams every iteration",
+            "design_achieved": design / per_launch_s / 1e9,
+            # the second roof: the SpMM's nnz * M masked adds against the
+            # vector ALUs' non-packed fp32 lane-add rate (256 CUs x 4 SIMD x
+            # 16 lanes x 2.4 GHz); this design counts them in bit planes
+            "valu_roof": {"ops_per_launch": nnz * info["M"],
+                          "achieved": nnz * info["M"] / per_launch_s / 1e12,
+                          "peak": VALU_LANE_ADDS_T, "unit": "T lane-adds/s",
+                          "frac": nnz * info["M"] / per_launch_s / 1e12 / VALU_LANE_ADDS_T}}
 
 
 def time_runs(sim, T: int, steps: int, warmup: int):
@@ -115,6 +156,7 @@ def bench_single(args, w) -> dict:
     N, M, T = info["n_nodes"], info["M"], w.params.iterations
     log(f"{w.name}: N={N} M={M} T={T} contacts={info['n_contacts']} windows={info['n_windows']} "
         f"epochs={info['n_epochs']} entries={info['n_entries']} open {time.time() - t0:.1f}s")
+    nnz, nnz_note = mean_nnz(sim, T)
     wall, kernel_ms = time_runs(sim, T, args.steps, args.warmup)
     counts = sim.counts()
     sim.close()
@@ -128,7 +170,7 @@ def bench_single(args, w) -> dict:
                    "window_s": w.params.static_network_duration,
                    "windows_per_epoch": info["windows_per_epoch"], "variant": info["variant"],
                    "final_recovered_mean": float(counts[-1, :, 2].mean())},
-        "roofline": roofline(info, per_launch, w.name),
+        "roofline": roofline(info, per_launch, w.name, nnz, nnz_note),
     }
     if not args.no_cpu_baseline:
         # bounded samples of about 10-30 s of oracle work (replicas, iterations)
@@ -151,6 +193,7 @@ def bench_batched(args, w) -> dict:
     variant = batch.autotune(min(T, 24))
     info = batch.info()
     N = info["n_nodes"]
+    nnz, nnz_note = mean_nnz(batch, T)
     log(f"{w.name}: {len(w.networks)} networks, N={N} M={M} T={T} contacts={info['n_contacts']} "
         f"epochs={info['n_epochs']} entries={info['n_entries']} variant={variant} "
         f"open {time.time() - t0:.1f}s")
@@ -169,7 +212,7 @@ def bench_batched(args, w) -> dict:
                    "windows_per_epoch": info["windows_per_epoch"], "variant": variant,
                    "hipgraph": not args.eager,
                    "final_recovered_mean": float(bcounts[-1, :, :, 2].mean())},
-        "roofline": roofline(info, per_launch, w.name),
+        "roofline": roofline(info, per_launch, w.name, nnz, nnz_note),
     }
     if args.no_plain:
         return out

@@ -26,7 +26,7 @@
 extern "C" {
 #endif
 
-#define REPLAY_HIP_ABI_VERSION 1
+#define REPLAY_HIP_ABI_VERSION 2
 
 enum {
   REPLAY_OK = 0,
@@ -68,6 +68,8 @@ typedef struct replay_params {
 /* replay_params.flags */
 #define REPLAY_F_GRAPH 1      /* full runs from iteration 0 replay one captured hipGraph */
 #define REPLAY_F_AUTOTUNE 2   /* time the kernel variants at open and keep the fastest */
+#define REPLAY_F_WIDE_ENTRIES 4 /* step kernels read the 16-byte packed entries even where
+                                   their 8-byte form fits (comparisons and measurement) */
 
 /* One network of a batch (replay_open_batch): its contacts, as in
    replay_open_contacts, and its own epidemiological parameters. */
@@ -106,6 +108,8 @@ typedef struct replay_info {
   int32_t n_networks;         /* networks in the context (1 unless batched) */
   int32_t variant;            /* step-kernel variant in use (REPLAY_V_*) */
   int32_t graph_ready;        /* a captured full-run hipGraph exists */
+  int32_t entry_bytes;        /* bytes per packed entry the step kernels stream:
+                                 8 (the 8-byte copy) or 16 */
 } replay_info;
 
 /* replay_info.variant */
@@ -211,7 +215,8 @@ int32_t replay_set_uniforms(replay_ctx *ctx, const float *u, int64_t n);
 /* CSR of iteration k's network window as build_csr_from_intervals
    (cu:238-350) would produce it, evaluated on the device from the packed
    slices.  rowptr[N+1]; colind/val hold up to cap entries; *nnz receives
-   the count (also when cap is too small, then REPLAY_E_ARG). */
+   the count (also when cap is too small, then REPLAY_E_ARG).  cap < 0
+   counts only: rowptr and *nnz are filled, colind/val may be NULL. */
 int32_t replay_get_slice(replay_ctx *ctx, int32_t k, int32_t *rowptr,
                          int32_t *colind, float *val, int64_t cap,
                          int64_t *nnz);

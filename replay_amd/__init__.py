@@ -23,7 +23,7 @@ CLI_PATH = os.path.join(PKG_DIR, "bin", "replay")
 GEN_PATH = os.path.join(PKG_DIR, "bin", "replay_gen")
 CGG_PATH = os.path.join(PKG_DIR, "bin", "replay_cgg")
 
-ABI_VERSION = 1
+ABI_VERSION = 2
 
 # error codes (include/replay_hip.h)
 E_ARG, E_IO, E_FORMAT, E_NODE_ID, E_DUP_ID, E_NO_EDGES, E_COUNT, E_NODE_RANGE = \
@@ -65,11 +65,12 @@ class _Info(C.Structure):
                 ("state_bytes", C.c_int64), ("steps_done", C.c_int32),
                 ("truncated_windows", C.c_int32), ("step_bytes", C.c_int64),
                 ("ring_depth", C.c_int32), ("n_networks", C.c_int32), ("variant", C.c_int32),
-                ("graph_ready", C.c_int32)]
+                ("graph_ready", C.c_int32), ("entry_bytes", C.c_int32)]
 
 
 F_GRAPH = 1
 F_AUTOTUNE = 2
+F_WIDE_ENTRIES = 4
 V_WORD, V_NODE4, V_NODE16, V_SLICED = 0, 4, 16, 1
 
 
@@ -383,6 +384,15 @@ class Simulation:
                                        _ptr(col, C.c_int32), _ptr(val, C.c_float), n,
                                        C.byref(nnz)))
         return rowptr, col[:n], val[:n]
+
+    def slice_nnz(self, k: int) -> int:
+        """Nonzeros of iteration k's window CSR (merged columns), counted on
+        the device without building the CSR."""
+        rowptr = np.zeros(self.n_nodes + 1, np.int32)
+        nnz = C.c_int64(0)
+        self._check(load_library().replay_get_slice(self._ctx, k, _ptr(rowptr, C.c_int32), None,
+                                                    None, -1, C.byref(nnz)))
+        return nnz.value
 
     def exposure(self, k: int) -> np.ndarray:
         """Exposure seconds (N*M, column-major) of the current state in window k."""

@@ -43,6 +43,7 @@ struct NetDev {
 struct StepArgs {
   const int32_t *rowptr;      // slice epoch rowptr (N+1), epoch-relative entries
   const int4 *entries;        // slice epoch entries {col, ts, te, rank}
+  const uint2 *entries8;      // the same entries as {col | ts << 24, ts >> 8 | te << 12}, or null
   const uint64_t *I_cur;      // infectious bits at iteration k
   uint64_t *I_next;           // infectious bits at iteration k+1
   uint64_t *S;                // susceptible bits, k -> k+1 in place
@@ -114,6 +115,7 @@ struct replay_ctx {
   hipGraphExec_t graph = nullptr;
   // device
   int4 *d_entries = nullptr;
+  uint2 *d_entries8 = nullptr;  // 8-byte copy of d_entries, where every entry fits one
   int32_t *d_rowptr = nullptr;
   uint64_t *d_S = nullptr;
   uint64_t *d_I[2] = {nullptr, nullptr};

@@ -48,6 +48,7 @@
 #include <cstring>
 #include <string>
 #include <thread>
+#include <type_traits>
 #include <vector>
 
 #include "engine.hpp"
@@ -216,6 +217,29 @@ __device__ __forceinline__ v4i load_chunk(const int4 *entries, int e, int e1, in
   return __builtin_nontemporal_load(reinterpret_cast<const v4i *>(entries + min(e + lane, e1 - 1)));
 }
 
+typedef unsigned int v2u __attribute__((ext_vector_type(2)));
+
+// The same chunk from the 8-byte copy of the entries.
+__device__ __forceinline__ v2u load_chunk(const uint2 *entries8, int e, int e1, int lane) {
+  return __builtin_nontemporal_load(reinterpret_cast<const v2u *>(entries8 + min(e + lane, e1 - 1)));
+}
+
+// Fields of a packed entry: {col, ts, te, rank} from the 16-byte form; from
+// the 8-byte form {col | ts << 24, ts >> 8 | te << 12} (24-bit column,
+// 20-bit epoch-relative times), whose contexts have no rank cap.
+struct EntryFields {
+  int col, ts, te;
+  uint32_t rank;
+};
+__device__ __forceinline__ EntryFields fields(const v4i &en) {
+  return EntryFields{en.x, en.y, en.z, static_cast<uint32_t>(en.w)};
+}
+__device__ __forceinline__ EntryFields fields(const v2u &en) {
+  return EntryFields{static_cast<int>(en.x & 0xFFFFFFu),
+                     static_cast<int>((en.x >> 24) | ((en.y & 0xFFFu) << 8)),
+                     static_cast<int>(en.y >> 12), 0u};
+}
+
 // Exposure of the words q0 .. q0+nq-1 (nq <= WT) of node i for one window.
 // The wavefront takes 64 contact entries at a time (one per lane, coalesced)
 // together with the neighbours' infectious words of the range, then takes
@@ -229,11 +253,11 @@ __device__ __forceinline__ v4i load_chunk(const int4 *entries, int e, int e1, in
 // SparseDenseProduct.h:64-70), visiting only entries that overlap the window
 // with a nonzero word: a column's entries share one neighbour, so a column is
 // skipped whole or not at all, and adding nothing leaves the fp32 sum as it is.
-template <bool kIntMode, int WT, bool kFull>
+template <bool kIntMode, int WT, bool kFull, typename EnT>
 __device__ __forceinline__ void group_exposure(const StepArgs &a, int e0, int e1,
                                                uint32_t rank_limit, int q0, int nq, int lane,
-                                               const uint64_t (&ss)[WT], const v4i &en0,
-                                               const v4i &en1, float (&E)[WT]) {
+                                               const uint64_t (&ss)[WT], const EnT &en0,
+                                               const EnT &en1, float (&E)[WT]) {
   uint32_t iacc[WT];
   float facc[WT];
 #pragma unroll
@@ -250,11 +274,12 @@ __device__ __forceinline__ void group_exposure(const StepArgs &a, int e0, int e1
   for (int q = 0; q < WT; ++q) cur_lo[q] = cur_hi[q] = 0;
   // overlap of this lane's entry of chunk c0 (0 when it does not count) and
   // the column whose words it reads (row 0 when it does not count)
-  auto decode = [&](int c0, const v4i &en, int &col, int &ov) {
-    int o = min(en.z, a.we_rel) - max(en.y, a.ws_rel);
-    if (!(c0 + lane < e1 && o > 0 && static_cast<uint32_t>(en.w) <= rank_limit)) o = 0;
+  auto decode = [&](int c0, const EnT &raw, int &col, int &ov) {
+    const EntryFields en = fields(raw);
+    int o = min(en.te, a.we_rel) - max(en.ts, a.ws_rel);
+    if (!(c0 + lane < e1 && o > 0 && en.rank <= rank_limit)) o = 0;
     ov = o;
-    col = o > 0 ? en.x : 0;
+    col = o > 0 ? en.col : 0;
   };
   // kFull: W is a multiple of WT, so every group is whole and, WT being
   // even, the row offset col * W + q0 is even and 16-byte aligned.  Such a
@@ -349,7 +374,14 @@ __device__ __forceinline__ void group_exposure(const StepArgs &a, int e0, int e1
   // pair is issued before the first is waited for, then the pair is reduced
   // and the next pair's entries are loaded.
   constexpr bool kPair = WT <= 4;
-  v4i enA = en0, enB = en1;
+  // the chunks come from the copy of the entries whose form EnT is
+  const auto *src = [&] {
+    if constexpr (sizeof(EnT) == 8)
+      return a.entries8;
+    else
+      return a.entries;
+  }();
+  EnT enA = en0, enB = en1;
   for (int c0 = e0; c0 < e1;) {
     int colA, ovA, colB = 0, ovB = 0;
     uint64_t rawA[WT], rawB[WT];
@@ -372,8 +404,8 @@ __device__ __forceinline__ void group_exposure(const StepArgs &a, int e0, int e1
     }
     c0 += kPair ? 2 * kWave : kWave;
     if (c0 < e1) {
-      enA = load_chunk(a.entries, c0, e1, lane);
-      if (kPair && c0 + kWave < e1) enB = load_chunk(a.entries, c0 + kWave, e1, lane);
+      enA = load_chunk(src, c0, e1, lane);
+      if (kPair && c0 + kWave < e1) enB = load_chunk(src, c0 + kWave, e1, lane);
     }
   }
   if (!kIntMode && cur_any) {
@@ -440,8 +472,11 @@ __device__ __forceinline__ uint64_t draw_word(const StepArgs &a, const NetDev &n
 // Phase 1 of an iteration: one wavefront per node computes the exposure of
 // its susceptible replicas, draws, and writes the node's infection words
 // (zero words included) into ring slot k.
-template <bool kIntMode, bool kInjected, int WT, bool kFull>
-__global__ __launch_bounds__(kBlock) void expo_kernel(StepArgs a) {
+template <bool kIntMode, bool kInjected, int WT, bool kFull, bool kCompact>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WT <= 4 ? 8 : 1)))
+void expo_kernel(StepArgs a) {
+  // kCompact: the chunks are read from the 8-byte copy of the entries
+  using EnT = std::conditional_t<kCompact, v2u, v4i>;
   const int lane = threadIdx.x & (kWave - 1);
   // Workgroups are dealt to the 8 XCDs round-robin; remap so that XCD x
   // takes the x-th contiguous eighth of the nodes.  Networks are disjoint
@@ -467,10 +502,15 @@ __global__ __launch_bounds__(kBlock) void expo_kernel(StepArgs a) {
     for (int w = lane; w < a.W; w += kWave) X_k[w] = 0ull;
     return;
   }
-  v4i en0 = v4i{0, 0, 0, 0}, en1 = v4i{0, 0, 0, 0};
+  EnT en0{}, en1{};
   if (e1 > e0) {
-    en0 = load_chunk(a.entries, e0, e1, lane);
-    if (WT <= 4 && e1 - e0 > kWave) en1 = load_chunk(a.entries, e0 + kWave, e1, lane);
+    if constexpr (kCompact) {
+      en0 = load_chunk(a.entries8, e0, e1, lane);
+      if (WT <= 4 && e1 - e0 > kWave) en1 = load_chunk(a.entries8, e0 + kWave, e1, lane);
+    } else {
+      en0 = load_chunk(a.entries, e0, e1, lane);
+      if (WT <= 4 && e1 - e0 > kWave) en1 = load_chunk(a.entries, e0 + kWave, e1, lane);
+    }
   }
   const NetDev nd = a.nets[net];
   const uint32_t rlim = a.rank_limit ? a.rank_limit[net] : UINT32_MAX;
@@ -492,7 +532,7 @@ __global__ __launch_bounds__(kBlock) void expo_kernel(StepArgs a) {
     for (int q = 0; q < WT; ++q) x[q] = 0ull;
     if (any_sus != 0ull && (e1 > e0 || kInjected)) {
       float E[WT];
-      group_exposure<kIntMode, WT, kFull>(a, e0, e1, rlim, q0, nq, lane, ss, en0, en1, E);
+      group_exposure<kIntMode, WT, kFull, EnT>(a, e0, e1, rlim, q0, nq, lane, ss, en0, en1, E);
       uint32_t pw[4] = {0u, 0u, 0u, 0u};
       int pw_grp = -1;
 #pragma unroll
@@ -535,7 +575,7 @@ __global__ __launch_bounds__(kBlock) void long_row_kernel(StepArgs a, const int3
     float E[1];
     const v4i en0 = load_chunk(a.entries, e0, e1, lane);  // rows here hold > 2 chunks
     const v4i en1 = load_chunk(a.entries, e0 + kWave, e1, lane);
-    group_exposure<kIntMode, 1, false>(a, e0, e1, rlim, q, 1, lane, ss, en0, en1, E);
+    group_exposure<kIntMode, 1, false, v4i>(a, e0, e1, rlim, q, 1, lane, ss, en0, en1, E);
     uint32_t pw[4] = {0u, 0u, 0u, 0u};
     int pw_grp = -1;
     x = draw_word<kInjected>(a, nd, net, i, q, lane, ss[0], E[0], pw, pw_grp);
@@ -882,6 +922,24 @@ __global__ __launch_bounds__(kBlock) void init_kernel(InitArgs a) {
 
 // P of an exposure of c whole windows, per network: the value the draw
 // would compute from E = c * window, 1 - exp(log(1-p) * (E / 3600)).
+// The packed entries in 8 bytes, {col | ts << 24, ts >> 8 | te << 12}: the
+// form the exposure kernels stream when every entry fits it.  An entry that
+// does not (a column of 2^24 or more, a time outside [0, 2^20)) sets
+// *misfit; the copy is then discarded.
+__global__ void pack8_kernel(const int4 *in, uint2 *out, int64_t n, int32_t *misfit) {
+  const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
+  for (int64_t e = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; e < n;
+       e += stride) {
+    const int4 en = in[e];
+    const bool fits = static_cast<uint32_t>(en.x) < (1u << 24) &&
+                      static_cast<uint32_t>(en.y) < (1u << 20) &&
+                      static_cast<uint32_t>(en.z) < (1u << 20);
+    if (!fits) atomicOr(misfit, 1);
+    out[e] = make_uint2(static_cast<uint32_t>(en.x) | (static_cast<uint32_t>(en.y) << 24),
+                        (static_cast<uint32_t>(en.y) >> 8) | (static_cast<uint32_t>(en.z) << 12));
+  }
+}
+
 __global__ void p_table_kernel(float *tab, const NetDev *nets, int32_t n_net, int32_t tab_max,
                                int32_t window) {
   const int g = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1002,7 +1060,7 @@ void release(replay_ctx *ctx) {
   if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
   if (ctx->graph) (void)hipGraphExecDestroy(ctx->graph);
   ctx->graph = nullptr;
-  void *bufs[] = {ctx->d_entries, ctx->d_rowptr, ctx->d_S,      ctx->d_I[0],        ctx->d_I[1],
+  void *bufs[] = {ctx->d_entries, ctx->d_entries8, ctx->d_rowptr, ctx->d_S,      ctx->d_I[0],        ctx->d_I[1],
                   ctx->d_X,       ctx->d_Xinit,  ctx->d_entry,  ctx->d_inf,         ctx->d_P,
                   ctx->d_init_count, ctx->d_counts, ctx->d_class_init, ctx->d_segs, ctx->d_nets,
                   ctx->d_net_of,  ctx->d_rank_limit, ctx->d_uniforms, ctx->d_p_tab,
@@ -1082,6 +1140,7 @@ StepArgs rp::step_args(replay_ctx *ctx, int32_t k) {
   std::memset(&a, 0, sizeof a);
   a.rowptr = ctx->d_rowptr + static_cast<size_t>(e) * (ctx->N + 1);
   a.entries = ctx->d_entries + ctx->epoch_entry0[e];
+  a.entries8 = ctx->d_entries8 ? ctx->d_entries8 + ctx->epoch_entry0[e] : nullptr;
   a.I_cur = ctx->d_I[ctx->cur];
   a.I_next = ctx->d_I[ctx->cur ^ 1];
   a.S = ctx->d_S;
@@ -1123,18 +1182,25 @@ void launch_expo(const StepArgs &a, int variant, hipStream_t st) {
   const dim3 grid((nb + kXcds - 1) / kXcds * kXcds);
   // whole groups (W a multiple of the group width) take the kernel whose
   // neighbour-word loads are 16 bytes wide
+  // and, where the context holds the 8-byte copy of the entries, the one
+  // that streams it
+  const bool c8 = a.entries8 != nullptr;
   if (variant == REPLAY_V_NODE16) {
-    if (a.W % 16 == 0)
-      hipLaunchKernelGGL((expo_kernel<kInt, kInj, 16, true>), grid, dim3(kBlock), 0, st, a);
+    if (a.W % 16 == 0 && c8)
+      hipLaunchKernelGGL((expo_kernel<kInt, kInj, 16, true, true>), grid, dim3(kBlock), 0, st, a);
+    else if (a.W % 16 == 0)
+      hipLaunchKernelGGL((expo_kernel<kInt, kInj, 16, true, false>), grid, dim3(kBlock), 0, st, a);
     else
-      hipLaunchKernelGGL((expo_kernel<kInt, kInj, 16, false>), grid, dim3(kBlock), 0, st, a);
+      hipLaunchKernelGGL((expo_kernel<kInt, kInj, 16, false, false>), grid, dim3(kBlock), 0, st, a);
   } else if (variant == REPLAY_V_NODE4) {
-    if (a.W % 4 == 0)
-      hipLaunchKernelGGL((expo_kernel<kInt, kInj, 4, true>), grid, dim3(kBlock), 0, st, a);
+    if (a.W % 4 == 0 && c8)
+      hipLaunchKernelGGL((expo_kernel<kInt, kInj, 4, true, true>), grid, dim3(kBlock), 0, st, a);
+    else if (a.W % 4 == 0)
+      hipLaunchKernelGGL((expo_kernel<kInt, kInj, 4, true, false>), grid, dim3(kBlock), 0, st, a);
     else
-      hipLaunchKernelGGL((expo_kernel<kInt, kInj, 4, false>), grid, dim3(kBlock), 0, st, a);
+      hipLaunchKernelGGL((expo_kernel<kInt, kInj, 4, false, false>), grid, dim3(kBlock), 0, st, a);
   } else {
-    hipLaunchKernelGGL((expo_kernel<kInt, kInj, 1, false>), grid, dim3(kBlock), 0, st, a);
+    hipLaunchKernelGGL((expo_kernel<kInt, kInj, 1, false, false>), grid, dim3(kBlock), 0, st, a);
   }
 }
 
@@ -1546,6 +1612,36 @@ int finish_open(replay_ctx *ctx) {
       (rc = dev_alloc(ctx, &ctx->d_init_count, G)) ||
       (rc = dev_alloc(ctx, &ctx->d_counts, (T + 1) * G * 4)))
     return rc;
+  // The 8-byte copy of the entries, for contexts without a rank cap.  It is
+  // optional: when the device has no room for it or an entry does not fit
+  // it, the step kernels read the 16-byte entries.
+  if (!ctx->any_cut && !(p.flags & REPLAY_F_WIDE_ENTRIES) && !entries.empty()) {
+    const int64_t n = static_cast<int64_t>(entries.size());
+    int32_t *d_misfit = nullptr;
+    if (hipMalloc(reinterpret_cast<void **>(&ctx->d_entries8), static_cast<size_t>(n) * 8) !=
+            hipSuccess ||
+        hipMalloc(reinterpret_cast<void **>(&d_misfit), sizeof(int32_t)) != hipSuccess) {
+      (void)hipGetLastError();  // out of room: not an error of the open
+      (void)hipFree(ctx->d_entries8);
+      ctx->d_entries8 = nullptr;
+    } else {
+      int32_t misfit = 0;
+      HIP_OR_FAIL(ctx, hipMemcpyAsync(d_misfit, &misfit, sizeof misfit, hipMemcpyHostToDevice,
+                                      ctx->stream));
+      const int64_t blocks = std::min<int64_t>((n + 255) / 256, 65536);
+      hipLaunchKernelGGL(pack8_kernel, dim3(static_cast<unsigned>(blocks)), dim3(256), 0,
+                         ctx->stream, ctx->d_entries, ctx->d_entries8, n, d_misfit);
+      HIP_OR_FAIL(ctx, hipGetLastError());
+      HIP_OR_FAIL(ctx, hipMemcpyAsync(&misfit, d_misfit, sizeof misfit, hipMemcpyDeviceToHost,
+                                      ctx->stream));
+      HIP_OR_FAIL(ctx, hipStreamSynchronize(ctx->stream));
+      (void)hipFree(d_misfit);
+      if (misfit) {
+        (void)hipFree(ctx->d_entries8);
+        ctx->d_entries8 = nullptr;
+      }
+    }
+  }
   // probabilities of whole-window exposures, computed by the device's own exp
   if (p.static_network_duration < (int64_t(1) << 24)) {
     const int32_t D = static_cast<int32_t>(p.static_network_duration);
@@ -1751,13 +1847,17 @@ int32_t replay_get_info(replay_ctx *ctx, replay_info *out) {
   const double ent = ctx->epoch_t0.empty() ? 0.0
                                            : static_cast<double>(ctx->n_entries) /
                                                  static_cast<double>(ctx->epoch_t0.size());
-  out->step_bytes = NW * 8 * (5 + ring_reads) + static_cast<int64_t>(16.0 * ent) +
+  // entries at the width the NODE4 / NODE16 kernels of whole groups stream
+  const bool c8 = ctx->d_entries8 && ((ctx->variant == REPLAY_V_NODE4 && ctx->W % 4 == 0) ||
+                                      (ctx->variant == REPLAY_V_NODE16 && ctx->W % 16 == 0));
+  out->step_bytes = NW * 8 * (5 + ring_reads) + static_cast<int64_t>((c8 ? 8.0 : 16.0) * ent) +
                     4 * (static_cast<int64_t>(ctx->N) + 1) +
                     12 * static_cast<int64_t>(ctx->n_net) * ctx->M;
   out->ring_depth = ctx->depth;
   out->n_networks = ctx->n_net;
   out->variant = ctx->variant;
   out->graph_ready = ctx->graph != nullptr;
+  out->entry_bytes = c8 ? 8 : 16;
   return REPLAY_OK;
 }
 
@@ -1945,6 +2045,11 @@ int32_t replay_get_slice(replay_ctx *ctx, int32_t k, int32_t *rowptr, int32_t *c
   rowptr[0] = 0;
   for (int32_t i = 0; i < ctx->N; ++i) rowptr[i + 1] = rowptr[i] + cnt[i];
   *nnz = rowptr[ctx->N];
+  if (cap < 0) {  // count only
+    (void)hipFree(d_cnt);
+    (void)hipFree(d_rp);
+    return REPLAY_OK;
+  }
   if (*nnz > cap || (*nnz > 0 && (!colind || !val))) {
     hipFree(d_cnt);
     hipFree(d_rp);

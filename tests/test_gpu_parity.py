@@ -155,17 +155,19 @@ def test_trajectory_parity(oracle, case, seed):
         assert counts[:, :, 1].max() > 0
 
 
+@pytest.mark.parametrize("flags", [0, replay_amd.F_WIDE_ENTRIES])
 @pytest.mark.parametrize("M", [1024, 1100])
 @pytest.mark.parametrize("step,D", [(600.0, 600), (1000.0, 3600)])
-def test_every_variant_matches_oracle_wide(oracle, M, step, D):
+def test_every_variant_matches_oracle_wide(oracle, M, step, D, flags):
     """Every exposure variant against the oracle at 16 and 18 words per node,
     in integer (window = step) and float windows: W = 16 takes the
     whole-group kernels of NODE4 and NODE16 (16-byte neighbour-word loads),
-    W = 18 their partial last group."""
+    reading the 8-byte copy of the entries or, with F_WIDE_ENTRIES, the
+    16-byte entries; W = 18 their partial last group."""
     n, a, b, s, e = _random_case(21, n=70, e=2000)
     p = Params(M=M, step_size=step, static_network_duration=D, iterations=50,
                initial_infected=0.1, infect_prob=0.3, exposed_duration=1200,
-               infectious_duration=5400, resistant_duration=3600, seed=17)
+               infectious_duration=5400, resistant_duration=3600, seed=17, flags=flags)
     g = oracle.OracleGraph.from_contacts(n, a, b, s, e)
     counts, state, _ = g.simulate(_to_oracle_params(oracle, p, threads=8), want_state=True)
     assert counts[:, :, 1].max() > 0
@@ -178,9 +180,10 @@ def test_every_variant_matches_oracle_wide(oracle, M, step, D):
             np.testing.assert_array_equal(sim.state(), state, err_msg=f"variant {v}")
 
 
-def test_injected_uniform_stream(oracle):
+@pytest.mark.parametrize("M", [70, 256])
+def test_injected_uniform_stream(oracle, M):
     n, a, b, s, e = _random_case(3, n=50, e=1200)
-    p = Params(M=70, step_size=600.0, static_network_duration=1800, iterations=80,
+    p = Params(M=M, step_size=600.0, static_network_duration=1800, iterations=80,
                initial_infected=0.1, infect_prob=0.3, exposed_duration=1200,
                infectious_duration=7200, resistant_duration=3600)
     rng = np.random.default_rng(5)
@@ -255,6 +258,46 @@ def test_draws_at_probability_ties(oracle, step, D):
             np.testing.assert_array_equal(sim.state(), state)
         results.append(counts)
     assert not np.array_equal(results[0], results[1])
+
+
+def test_entry_forms(oracle):
+    """The step kernels stream the 8-byte copy of the entries where every
+    entry fits it (step_bytes counts 8 bytes an entry), and the 16-byte
+    entries where an epoch spans 2^20 s or more (epoch-relative times do not
+    fit 20 bits) or when F_WIDE_ENTRIES asks for them; every form gives the
+    oracle's trajectory."""
+    n, a, b, s, e = _random_case(5, n=60, e=1500, span=40 * 86400, long_frac=0.5)
+    def params(wpe, flags):
+        return Params(M=256, step_size=3600.0, static_network_duration=3600, iterations=400,
+                      initial_infected=0.05, infect_prob=0.2, exposed_duration=3600,
+                      infectious_duration=3 * 86400, resistant_duration=86400, seed=3,
+                      windows_per_epoch=wpe, flags=flags)
+    g = oracle.OracleGraph.from_contacts(n, a, b, s, e)
+    counts, state, _ = g.simulate(_to_oracle_params(oracle, params(8, 0)), want_state=True)
+    assert counts[:, :, 1].max() > 0
+    assert 400 * 3600 >= 1 << 20  # the one epoch's times need more than 20 bits
+    rest = set()
+    for wpe, flags, width in ((8, 0, 8), (8, replay_amd.F_WIDE_ENTRIES, 16), (400, 0, 16)):
+        with Simulation.from_contacts(n, a, b, s, e, params(wpe, flags)) as sim:
+            sim.set_variant(replay_amd.V_NODE4)
+            info = sim.info()
+            assert info["entry_bytes"] == width, (wpe, flags)
+            sim.run()
+            np.testing.assert_array_equal(sim.counts(), counts, err_msg=f"wpe {wpe} {flags}")
+            np.testing.assert_array_equal(sim.state(), state, err_msg=f"wpe {wpe} {flags}")
+            # step_bytes prices the entries at the streamed width; the rest
+            # (state words, row pointers, counts) does not depend on it
+            rest.add(info["step_bytes"] - int(width * (info["n_entries"] / info["n_epochs"])))
+    assert len(rest) == 1
+
+
+def test_slice_nnz_counts_the_window_csr():
+    n, a, b, s, e = _random_case(8, n=60, e=1500)
+    p = Params(M=64, step_size=1000.0, static_network_duration=3600, iterations=40)
+    with Simulation.from_contacts(n, a, b, s, e, p) as sim:
+        for k in range(0, p.iterations, 3):
+            rp, col, _ = sim.slice(k)
+            assert sim.slice_nnz(k) == len(col) == rp[-1]
 
 
 def test_truncated_windows_trajectory(oracle):
