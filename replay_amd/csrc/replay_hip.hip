@@ -301,6 +301,7 @@ __device__ __forceinline__ void group_exposure(const StepArgs &a, int e0, int e1
       for (int q = 0; q < WT; ++q) raw[q] = mrow[q < nq ? q : nq - 1];
     }
   };
+  // float mode: the words of entries that count, zero elsewhere
   auto mask = [&](int ov, const uint64_t (&raw)[WT], uint32_t (&lo)[WT], uint32_t (&hi)[WT]) {
 #pragma unroll
     for (int q = 0; q < WT; ++q) {
@@ -310,29 +311,35 @@ __device__ __forceinline__ void group_exposure(const StepArgs &a, int e0, int e1
     }
   };
   // one chunk of 64 entries: this lane's entry has column col, overlap ov
-  // and words (mlo, mhi), zero where the entry does not count
-  auto reduce = [&](int col, int ov, const uint32_t (&mlo)[WT], const uint32_t (&mhi)[WT]) {
+  // (0 where the entry does not count) and gathered words raw
+  auto reduce = [&](int col, int ov, const uint64_t (&raw)[WT]) {
     if (kIntMode) {
       // Whole-window contacts: count, per replica lane, the contacts whose
       // neighbour word has that replica's bit set (column counts of the
       // 64 x 64 block of gathered words); partial overlaps are added one
-      // contact at a time from the broadcast word.
+      // contact at a time from the broadcast word.  The words are not
+      // zeroed where the entry does not count: every lane read below is in
+      // `counting` (ov > 0), and a whole-window lane has ov == L > 0.
       const int L = a.we_rel - a.ws_rel;
-      const bool full = ov == L;
+      const bool full = L > 0 && ov == L;
+      const uint64_t counting = __ballot(ov > 0);
       const uint64_t partial = __ballot(ov > 0 && ov != L);
 #pragma unroll
       for (int q = 0; q < WT; ++q) {
-        const uint64_t m = (static_cast<uint64_t>(mhi[q]) << 32) | mlo[q];
-        const uint64_t nz = __ballot(m != 0ull);
         // words without susceptible replicas need no exposure
-        if (q >= nq || nz == 0ull || ss[q] == 0ull) continue;
+        if (q >= nq || ss[q] == 0ull) continue;
+        const uint64_t m = raw[q];
+        const uint64_t nz = __ballot(m != 0ull) & counting;
+        if (nz == 0ull) continue;
         // few contributing contacts: add them one at a time
         const bool few = __popcll(nz & ~partial) <= kFewContacts;
         const uint32_t c = (!few && (nz & ~partial)) ? column_count(full ? m : 0ull, lane) : 0u;
         for (uint64_t pm = few ? nz : (partial & nz); pm != 0ull; pm &= pm - 1) {
           const int j = __builtin_ctzll(pm);
-          const uint32_t lo = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(mlo[q]), j));
-          const uint32_t hi = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(mhi[q]), j));
+          const uint32_t lo = static_cast<uint32_t>(
+              __builtin_amdgcn_readlane(static_cast<int>(static_cast<uint32_t>(m)), j));
+          const uint32_t hi = static_cast<uint32_t>(
+              __builtin_amdgcn_readlane(static_cast<int>(static_cast<uint32_t>(m >> 32)), j));
           const uint64_t mj = (static_cast<uint64_t>(hi) << 32) | lo;
           const uint32_t ovj = static_cast<uint32_t>(__builtin_amdgcn_readlane(ov, j));
           iacc[q] += static_cast<uint32_t>((mj >> lane) & 1ull) * ovj;
@@ -340,6 +347,8 @@ __device__ __forceinline__ void group_exposure(const StepArgs &a, int e0, int e1
         iacc[q] += static_cast<uint32_t>(L) * c;
       }
     } else {
+      uint32_t mlo[WT], mhi[WT];
+      mask(ov, raw, mlo, mhi);
       uint64_t any = 0ull;
 #pragma unroll
       for (int q = 0; q < WT; ++q) any |= (static_cast<uint64_t>(mhi[q]) << 32) | mlo[q];
@@ -392,16 +401,8 @@ __device__ __forceinline__ void group_exposure(const StepArgs &a, int e0, int e1
       decode(c0 + kWave, enB, colB, ovB);
       fetch(colB, rawB);
     }
-    {
-      uint32_t lo[WT], hi[WT];
-      mask(ovA, rawA, lo, hi);
-      reduce(colA, ovA, lo, hi);
-    }
-    if (two) {
-      uint32_t lo[WT], hi[WT];
-      mask(ovB, rawB, lo, hi);
-      reduce(colB, ovB, lo, hi);
-    }
+    reduce(colA, ovA, rawA);
+    if (two) reduce(colB, ovB, rawB);
     c0 += kPair ? 2 * kWave : kWave;
     if (c0 < e1) {
       enA = load_chunk(src, c0, e1, lane);
