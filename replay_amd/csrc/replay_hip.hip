@@ -73,6 +73,14 @@ constexpr int kFewContacts = REPLAY_FEW_CONTACTS;
 constexpr float kDecideMargin = 1.0e-5f;
 
 
+// Device buffers freed when the scope ends, whichever way it is left.
+struct DeviceScratch {
+  std::vector<void *> bufs;
+  ~DeviceScratch() {
+    for (void *b : bufs) (void)hipFree(b);
+  }
+};
+
 __device__ __forceinline__ const uint64_t *ring_word(const StepArgs &a, int32_t n, int64_t w) {
   // X(k+1-n) for n >= 2 (n == 1 is the local infection word)
   const int32_t e = a.k + 1 - n;
@@ -1619,6 +1627,7 @@ int finish_open(replay_ctx *ctx) {
   if (!ctx->any_cut && !(p.flags & REPLAY_F_WIDE_ENTRIES) && !entries.empty()) {
     const int64_t n = static_cast<int64_t>(entries.size());
     int32_t *d_misfit = nullptr;
+    DeviceScratch scratch;
     if (hipMalloc(reinterpret_cast<void **>(&ctx->d_entries8), static_cast<size_t>(n) * 8) !=
             hipSuccess ||
         hipMalloc(reinterpret_cast<void **>(&d_misfit), sizeof(int32_t)) != hipSuccess) {
@@ -1626,6 +1635,7 @@ int finish_open(replay_ctx *ctx) {
       (void)hipFree(ctx->d_entries8);
       ctx->d_entries8 = nullptr;
     } else {
+      scratch.bufs.push_back(d_misfit);
       int32_t misfit = 0;
       HIP_OR_FAIL(ctx, hipMemcpyAsync(d_misfit, &misfit, sizeof misfit, hipMemcpyHostToDevice,
                                       ctx->stream));
@@ -1636,7 +1646,6 @@ int finish_open(replay_ctx *ctx) {
       HIP_OR_FAIL(ctx, hipMemcpyAsync(&misfit, d_misfit, sizeof misfit, hipMemcpyDeviceToHost,
                                       ctx->stream));
       HIP_OR_FAIL(ctx, hipStreamSynchronize(ctx->stream));
-      (void)hipFree(d_misfit);
       if (misfit) {
         (void)hipFree(ctx->d_entries8);
         ctx->d_entries8 = nullptr;
@@ -2032,10 +2041,12 @@ int32_t replay_get_slice(replay_ctx *ctx, int32_t k, int32_t *rowptr, int32_t *c
   StepArgs a = step_args(ctx, k);
   int32_t *d_rp = nullptr, *d_cnt = nullptr, *d_col = nullptr;
   float *d_val = nullptr;
+  DeviceScratch scratch;
   int rc;
-  if ((rc = dev_alloc(ctx, &d_cnt, static_cast<size_t>(ctx->N))) ||
-      (rc = dev_alloc(ctx, &d_rp, static_cast<size_t>(ctx->N) + 1)))
-    return rc;
+  if ((rc = dev_alloc(ctx, &d_cnt, static_cast<size_t>(ctx->N)))) return rc;
+  scratch.bufs.push_back(d_cnt);
+  if ((rc = dev_alloc(ctx, &d_rp, static_cast<size_t>(ctx->N) + 1))) return rc;
+  scratch.bufs.push_back(d_rp);
   const int tb = 256, nb = (ctx->N + tb - 1) / tb;
   hipLaunchKernelGGL(slice_kernel, dim3(nb), dim3(tb), 0, ctx->stream, a, 0, d_rp, d_cnt,
                      nullptr, nullptr);
@@ -2046,19 +2057,13 @@ int32_t replay_get_slice(replay_ctx *ctx, int32_t k, int32_t *rowptr, int32_t *c
   rowptr[0] = 0;
   for (int32_t i = 0; i < ctx->N; ++i) rowptr[i + 1] = rowptr[i] + cnt[i];
   *nnz = rowptr[ctx->N];
-  if (cap < 0) {  // count only
-    (void)hipFree(d_cnt);
-    (void)hipFree(d_rp);
-    return REPLAY_OK;
-  }
-  if (*nnz > cap || (*nnz > 0 && (!colind || !val))) {
-    hipFree(d_cnt);
-    hipFree(d_rp);
+  if (cap < 0) return REPLAY_OK;  // count only
+  if (*nnz > cap || (*nnz > 0 && (!colind || !val)))
     return fail(ctx, REPLAY_E_ARG, "slice buffer too small");
-  }
-  if ((rc = dev_alloc(ctx, &d_col, static_cast<size_t>(*nnz))) ||
-      (rc = dev_alloc(ctx, &d_val, static_cast<size_t>(*nnz))))
-    return rc;
+  if ((rc = dev_alloc(ctx, &d_col, static_cast<size_t>(*nnz)))) return rc;
+  scratch.bufs.push_back(d_col);
+  if ((rc = dev_alloc(ctx, &d_val, static_cast<size_t>(*nnz)))) return rc;
+  scratch.bufs.push_back(d_val);
   HIP_OR_FAIL(ctx, hipMemcpyAsync(d_rp, rowptr, (ctx->N + 1) * sizeof(int32_t),
                                   hipMemcpyHostToDevice, ctx->stream));
   hipLaunchKernelGGL(slice_kernel, dim3(nb), dim3(tb), 0, ctx->stream, a, 1, d_rp, d_cnt, d_col,
@@ -2070,10 +2075,6 @@ int32_t replay_get_slice(replay_ctx *ctx, int32_t k, int32_t *rowptr, int32_t *c
                                     ctx->stream));
   }
   HIP_OR_FAIL(ctx, hipStreamSynchronize(ctx->stream));
-  hipFree(d_cnt);
-  hipFree(d_rp);
-  hipFree(d_col);
-  hipFree(d_val);
   return REPLAY_OK;
 }
 
@@ -2086,6 +2087,7 @@ int32_t replay_get_exposure(replay_ctx *ctx, int32_t k, float *out, int64_t n_ou
   float *d_out = nullptr;
   int rc = dev_alloc(ctx, &d_out, static_cast<size_t>(n));
   if (rc) return rc;
+  DeviceScratch scratch{{d_out}};
   dim3 grid((ctx->N + kWavesPerBlock - 1) / kWavesPerBlock, ctx->W);
   if (ctx->win_int[ctx->iter_window[k]])
     hipLaunchKernelGGL(exposure_kernel<true>, grid, dim3(kBlock), 0, ctx->stream, a, d_out);
@@ -2095,7 +2097,6 @@ int32_t replay_get_exposure(replay_ctx *ctx, int32_t k, float *out, int64_t n_ou
   HIP_OR_FAIL(ctx, hipMemcpyAsync(out, d_out, static_cast<size_t>(n) * sizeof(float),
                                   hipMemcpyDeviceToHost, ctx->stream));
   HIP_OR_FAIL(ctx, hipStreamSynchronize(ctx->stream));
-  hipFree(d_out);
   return REPLAY_OK;
 }
 
