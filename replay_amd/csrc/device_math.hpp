@@ -40,6 +40,12 @@ __device__ __forceinline__ uint32_t philox_word0(uint64_t seed, uint32_t c0, uin
 __device__ __forceinline__ void philox4(uint64_t seed, uint32_t c0, uint32_t c1, uint32_t c2,
                                         uint32_t c3, uint32_t (&out)[4]) {
   uint32_t k0 = static_cast<uint32_t>(seed), k1 = static_cast<uint32_t>(seed >> 32);
+  // The key is the same for every lane and lives in SGPRs.  Made opaque
+  // here, its schedule is computed by the scalar unit where a block is
+  // drawn, instead of being hoisted to the kernel's start and held in VGPR
+  // lanes until use (a spill and a reload, both vector instructions, per
+  // round key).
+  asm volatile("" : "+s"(k0), "+s"(k1));
 #pragma unroll
   for (int r = 0; r < 10; ++r) {
     const uint64_t p0 = static_cast<uint64_t>(0xD2511F53u) * c0;

@@ -439,19 +439,27 @@ __device__ __forceinline__ uint64_t draw_word(const StepArgs &a, const NetDev &n
                                               uint32_t (&pw)[4], int &pw_grp) {
   const int r = Q * kWave + lane;
   const bool sus = (ss >> lane) & 1ull;
-  bool infect = false;
   // E == 0 gives P = 0 (or NaN), never above a uniform in [0, 1)
-  if (sus && (kInjected || E > 0.0f)) {
+  const bool need = sus && (kInjected || E > 0.0f);
+  // The block serving words 4g .. 4g+3 is computed once per wavefront, by
+  // every lane together, when the first word of the group that some lane
+  // draws comes up (a uniform branch; pw_grp is the wavefront's).  Counter
+  // words 0, 2, 3 are the wavefront's (node, slot, stream): made opaque,
+  // the products of the first round are formed here by the scalar unit
+  // rather than hoisted to the kernel's start and held in VGPR lanes.
+  if (!kInjected && pw_grp != (Q >> 2) && __ballot(need) != 0ull) {
+    uint32_t c0 = static_cast<uint32_t>(ii - nd.node_off), c2 = a.slot, c3 = nd.stream;
+    asm volatile("" : "+s"(c0), "+s"(c2), "+s"(c3));
+    rp::philox4(a.seed, c0, (static_cast<uint32_t>(Q >> 2) << 6) | static_cast<uint32_t>(lane),
+                c2, c3, pw);
+    pw_grp = Q >> 2;
+  }
+  bool infect = false;
+  if (need) {
     float u;
     if (kInjected) {
       u = a.uniforms[static_cast<size_t>(r) * a.N + ii];
     } else {
-      if (pw_grp != (Q >> 2)) {
-        rp::philox4(a.seed, static_cast<uint32_t>(ii - nd.node_off),
-                    (static_cast<uint32_t>(Q >> 2) << 6) | static_cast<uint32_t>(lane), a.slot,
-                    nd.stream, pw);
-        pw_grp = Q >> 2;
-      }
       const int k = Q & 3;
       u = rp::u24(k == 0 ? pw[0] : k == 1 ? pw[1] : k == 2 ? pw[2] : pw[3]);
     }
