@@ -60,7 +60,8 @@ typedef struct replay_params {
   uint64_t seed;              /* Philox4x32-10 key (new: the reference seeds by clock) */
   int32_t device;             /* HIP device ordinal */
   int32_t windows_per_epoch;  /* slice packing; <= 0 picks automatically */
-  int64_t max_slice_bytes;    /* device budget for packed slices; <= 0: 8 GiB */
+  int64_t max_slice_bytes;    /* device budget for packed 16-byte slices (the 8-byte
+                                 copy adds half again); <= 0: 32 GiB */
   uint32_t stream;            /* Philox stream id of this network (counter word 3) */
   int32_t flags;              /* REPLAY_F_* */
 } replay_params;

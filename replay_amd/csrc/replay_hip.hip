@@ -1415,7 +1415,11 @@ void parallel_for(size_t n, F fn) {
 // Picks the epoch size for all networks together.
 int32_t pick_epoch(replay_ctx *ctx) {
   if (ctx->p.windows_per_epoch > 0) return ctx->p.windows_per_epoch;
-  const int64_t budget = ctx->p.max_slice_bytes > 0 ? ctx->p.max_slice_bytes : (int64_t(8) << 30);
+  // 32 GiB of 16-byte entries by default (48 GiB with the 8-byte copy) of
+  // the 288 GB: shorter epochs stream fewer entries per step (measured:
+  // 32 -> 8 windows per epoch, -2.7% per step on the 64-network workload;
+  // 128 -> 32 on the dense one, -8.7%), at the cost of packing more.
+  const int64_t budget = ctx->p.max_slice_bytes > 0 ? ctx->p.max_slice_bytes : (int64_t(32) << 30);
   const int32_t W = static_cast<int32_t>(ctx->win_ws.size());
   std::vector<int64_t> cand;
   for (int64_t S = 1; S < W; S *= 2) cand.push_back(S);
