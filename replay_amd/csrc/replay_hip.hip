@@ -490,7 +490,12 @@ __device__ __forceinline__ uint64_t draw_word(const StepArgs &a, const NetDev &n
 // its susceptible replicas, draws, and writes the node's infection words
 // (zero words included) into ring slot k.
 template <bool kIntMode, bool kInjected, int WT, bool kFull, bool kCompact>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WT <= 4 ? 8 : 1)))
+// Occupancy floors, measured: 8 wavefronts per SIMD for groups of up to 4
+// words (SGPR values parked in VGPR lanes, no scratch); 6 for whole 16-word
+// groups (a few VGPRs to scratch, outside the chunk loop: -2.3% per step on
+// the dense workload); the compiler's choice otherwise.
+__global__ __launch_bounds__(kBlock) __attribute__((
+    amdgpu_waves_per_eu(WT <= 4 ? 8 : (WT == 16 && kFull && !kInjected ? 6 : 1))))
 void expo_kernel(StepArgs a) {
   // kCompact: the chunks are read from the 8-byte copy of the entries
   using EnT = std::conditional_t<kCompact, v2u, v4i>;
