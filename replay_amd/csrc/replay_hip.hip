@@ -574,9 +574,16 @@ void expo_kernel(StepArgs a) {
 // Phase 1 for long rows (more than long_row entries in the epoch, e.g. the
 // hubs of a power-law network): one wavefront per (listed row, 64-replica
 // word), so a hub's contact list is walked by W wavefronts in parallel
-// instead of one.  Integer mode only where the row's own weight bound is
-// below 2^24; otherwise the float path, which follows the reference order.
-template <bool kIntMode, bool kInjected>
+// instead of one.  Integer mode where the row's own weight bound is below
+// 2^24; otherwise the float path, which follows the reference order.
+// kTryInt (rows of a window whose weights are integers, divisor 1, but whose
+// bound reaches 2^24): the row is first summed in integers.  Weights are
+// non-negative, so while a replica's sum stays below 2^24 every partial sum
+// of the reference's fp32 order is an integer that fp32 holds exactly, and
+// the integer sum is that order's result.  Only when a susceptible
+// replica's sum reaches 2^24, or the row could overflow 32 bits, is it
+// summed again in the reference order.
+template <bool kIntMode, bool kInjected, bool kTryInt>
 __global__ __launch_bounds__(kBlock) void long_row_kernel(StepArgs a, const int32_t *rows,
                                                           int n_rows) {
   const int lane = threadIdx.x & (kWave - 1);
@@ -597,7 +604,16 @@ __global__ __launch_bounds__(kBlock) void long_row_kernel(StepArgs a, const int3
     float E[1];
     const v4i en0 = load_chunk(a.entries, e0, e1, lane);  // rows here hold > 2 chunks
     const v4i en1 = load_chunk(a.entries, e0 + kWave, e1, lane);
-    group_exposure<kIntMode, 1, false, v4i>(a, e0, e1, rlim, q, 1, lane, ss, en0, en1, E);
+    bool done = false;
+    if (kTryInt && static_cast<int64_t>(e1 - e0) * (a.we_rel - a.ws_rel) < (int64_t(1) << 32)) {
+      group_exposure<true, 1, false, v4i>(a, e0, e1, rlim, q, 1, lane, ss, en0, en1, E);
+      // E is the integer sum converted to float: >= 2^24 exactly when the
+      // sum is
+      const bool sus = (ss[0] >> lane) & 1ull;
+      done = __ballot(sus && E[0] >= 16777216.0f) == 0ull;
+    }
+    if (!done)
+      group_exposure<kIntMode, 1, false, v4i>(a, e0, e1, rlim, q, 1, lane, ss, en0, en1, E);
     uint32_t pw[4] = {0u, 0u, 0u, 0u};
     int pw_grp = -1;
     x = draw_word<kInjected>(a, nd, net, i, q, lane, ss[0], E[0], pw, pw_grp);
@@ -1255,21 +1271,24 @@ struct LongRows {
   int32_t n_int, n_all;
 };
 
-template <bool kInt, bool kInj>
+template <bool kInt, bool kInj, bool kTryInt>
 void launch_long_part(const StepArgs &a, const int32_t *rows, int32_t n, hipStream_t st) {
   if (n <= 0) return;
   const int64_t waves = static_cast<int64_t>(n) * a.W;
   const dim3 grid(static_cast<unsigned>((waves + kWavesPerBlock - 1) / kWavesPerBlock));
-  hipLaunchKernelGGL((long_row_kernel<kInt, kInj>), grid, dim3(kBlock), 0, st, a, rows, n);
+  hipLaunchKernelGGL((long_row_kernel<kInt, kInj, kTryInt>), grid, dim3(kBlock), 0, st, a, rows,
+                     n);
 }
 
+// In an integer window (divisor 1) the rows beyond the 2^24 bound have
+// integer weights too, and try the integer sum first.
 template <bool kInj>
 void launch_long(const StepArgs &a, bool int_mode, const LongRows &lr, hipStream_t st) {
   if (int_mode) {
-    launch_long_part<true, kInj>(a, lr.rows, lr.n_int, st);
-    launch_long_part<false, kInj>(a, lr.rows + lr.n_int, lr.n_all - lr.n_int, st);
+    launch_long_part<true, kInj, false>(a, lr.rows, lr.n_int, st);
+    launch_long_part<false, kInj, true>(a, lr.rows + lr.n_int, lr.n_all - lr.n_int, st);
   } else {
-    launch_long_part<false, kInj>(a, lr.rows, lr.n_all, st);
+    launch_long_part<false, kInj, false>(a, lr.rows, lr.n_all, st);
   }
 }
 

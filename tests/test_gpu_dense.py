@@ -124,6 +124,26 @@ def test_long_rows_match_oracle(oracle, variant):
     np.testing.assert_array_equal(state.reshape(-1), ostate)
 
 
+def test_long_row_sums_beyond_2_24_match_oracle(oracle):
+    """The 5000-leaf hub's row is listed beyond the 2^24 bound and is summed
+    in integers first.  Here nearly every leaf is seeded infected, so from
+    the second iteration a susceptible hub replica's sum (about 4850 leaves x
+    3600 s) passes 2^24 and the row is summed again in the reference order;
+    the small infection probability keeps the hub's draws near P = 0.4, far
+    from saturating."""
+    n, a, b, s, e = _two_hubs(seed=4)
+    p = Params(M=256, step_size=3600.0, static_network_duration=3600, iterations=10,
+               initial_infected=0.97, infect_prob=1e-4, exposed_duration=0,
+               infectious_duration=20 * 3600, resistant_duration=10 ** 6, seed=9)
+    with Simulation.from_contacts(n, a, b, s, e, p) as sim:
+        sim.run()
+        counts, state = sim.counts(), sim.state()
+    g = oracle.OracleGraph.from_contacts(n, a, b, s, e)
+    oc, ostate, _ = g.simulate(_to_oracle_params(oracle, p, threads=8), want_state=True)
+    np.testing.assert_array_equal(counts, oc)
+    np.testing.assert_array_equal(state.reshape(-1), ostate)
+
+
 def test_long_row_exposure_matches_oracle_order(oracle):
     """Exposure of hub rows (integer-capable and beyond 2^24 seconds) equals
     the sequential fp32 sum of the reference CSR bit for bit."""
