@@ -428,6 +428,18 @@ __device__ __forceinline__ void group_exposure(const StepArgs &a, int e0, int e1
   for (int q = 0; q < WT; ++q) E[q] = kIntMode ? static_cast<float>(iacc[q]) : facc[q];
 }
 
+// Whether rows e0 .. e1 of a window of integer weights (divisor 1) may be
+// summed in integers first: every overlap below 2^24 (so the reference's
+// (float)overlap is exact) and no 32-bit sum overflow.  Weights being
+// non-negative, while a replica's sum stays below 2^24 every partial sum of
+// the reference's fp32 order is an integer that fp32 holds exactly, so the
+// integer sum is that order's result; the caller falls back to the
+// reference order when a susceptible replica's sum reaches 2^24.
+__device__ __forceinline__ bool try_integer_sums(const StepArgs &a, int e0, int e1) {
+  const int64_t L = static_cast<int64_t>(a.we_rel) - a.ws_rel;
+  return L < (int64_t(1) << 24) && static_cast<int64_t>(e1 - e0) * L < (int64_t(1) << 32);
+}
+
 // Draw of one word (word Q of node ii) given its lanes' exposure; records the
 // infections and returns their ballot.  The probability is computed, not
 // looked up: a table read is one more dependent memory round trip per node,
@@ -489,11 +501,17 @@ __device__ __forceinline__ uint64_t draw_word(const StepArgs &a, const NetDev &n
 // Phase 1 of an iteration: one wavefront per node computes the exposure of
 // its susceptible replicas, draws, and writes the node's infection words
 // (zero words included) into ring slot k.
-template <bool kIntMode, bool kInjected, int WT, bool kFull, bool kCompact>
+//
 // Occupancy floors, measured: 8 wavefronts per SIMD for groups of up to 4
 // words (SGPR values parked in VGPR lanes, no scratch); 6 for whole 16-word
 // groups (a few VGPRs to scratch, outside the chunk loop: -2.3% per step on
 // the dense workload); the compiler's choice otherwise.
+//
+// kTryInt (with kIntMode false: a window of integer weights, divisor 1,
+// whose short-row weight bound reaches 2^24): each group is summed in
+// integers first and in the reference order only when a susceptible
+// replica's sum reaches 2^24 (see try_integer_sums).
+template <bool kIntMode, bool kInjected, int WT, bool kFull, bool kCompact, bool kTryInt>
 __global__ __launch_bounds__(kBlock) __attribute__((
     amdgpu_waves_per_eu(WT <= 4 ? 8 : (WT == 16 && kFull && !kInjected ? 6 : 1))))
 void expo_kernel(StepArgs a) {
@@ -554,7 +572,18 @@ void expo_kernel(StepArgs a) {
     for (int q = 0; q < WT; ++q) x[q] = 0ull;
     if (any_sus != 0ull && (e1 > e0 || kInjected)) {
       float E[WT];
-      group_exposure<kIntMode, WT, kFull, EnT>(a, e0, e1, rlim, q0, nq, lane, ss, en0, en1, E);
+      bool done = false;
+      if constexpr (kTryInt) {
+        if (try_integer_sums(a, e0, e1)) {
+          group_exposure<true, WT, kFull, EnT>(a, e0, e1, rlim, q0, nq, lane, ss, en0, en1, E);
+          bool big = false;
+#pragma unroll
+          for (int q = 0; q < WT; ++q) big |= ((ss[q] >> lane) & 1ull) && E[q] >= 16777216.0f;
+          done = __ballot(big) == 0ull;
+        }
+      }
+      if (!done)
+        group_exposure<kIntMode, WT, kFull, EnT>(a, e0, e1, rlim, q0, nq, lane, ss, en0, en1, E);
       uint32_t pw[4] = {0u, 0u, 0u, 0u};
       int pw_grp = -1;
 #pragma unroll
@@ -605,7 +634,7 @@ __global__ __launch_bounds__(kBlock) void long_row_kernel(StepArgs a, const int3
     const v4i en0 = load_chunk(a.entries, e0, e1, lane);  // rows here hold > 2 chunks
     const v4i en1 = load_chunk(a.entries, e0 + kWave, e1, lane);
     bool done = false;
-    if (kTryInt && static_cast<int64_t>(e1 - e0) * (a.we_rel - a.ws_rel) < (int64_t(1) << 32)) {
+    if (kTryInt && try_integer_sums(a, e0, e1)) {
       group_exposure<true, 1, false, v4i>(a, e0, e1, rlim, q, 1, lane, ss, en0, en1, E);
       // E is the integer sum converted to float: >= 2^24 exactly when the
       // sum is
@@ -1214,8 +1243,8 @@ StepArgs rp::step_args(replay_ctx *ctx, int32_t k) {
 
 namespace {
 
-template <bool kInt, bool kInj>
-void launch_expo(const StepArgs &a, int variant, hipStream_t st) {
+template <bool kInt, bool kInj, bool kTry>
+void launch_expo_t(const StepArgs &a, int variant, hipStream_t st) {
   const unsigned nb = (a.N + kWavesPerBlock - 1) / kWavesPerBlock;
   const dim3 grid((nb + kXcds - 1) / kXcds * kXcds);
   // whole groups (W a multiple of the group width) take the kernel whose
@@ -1225,21 +1254,34 @@ void launch_expo(const StepArgs &a, int variant, hipStream_t st) {
   const bool c8 = a.entries8 != nullptr;
   if (variant == REPLAY_V_NODE16) {
     if (a.W % 16 == 0 && c8)
-      hipLaunchKernelGGL((expo_kernel<kInt, kInj, 16, true, true>), grid, dim3(kBlock), 0, st, a);
+      hipLaunchKernelGGL((expo_kernel<kInt, kInj, 16, true, true, kTry>), grid, dim3(kBlock), 0, st, a);
     else if (a.W % 16 == 0)
-      hipLaunchKernelGGL((expo_kernel<kInt, kInj, 16, true, false>), grid, dim3(kBlock), 0, st, a);
+      hipLaunchKernelGGL((expo_kernel<kInt, kInj, 16, true, false, kTry>), grid, dim3(kBlock), 0, st, a);
     else
-      hipLaunchKernelGGL((expo_kernel<kInt, kInj, 16, false, false>), grid, dim3(kBlock), 0, st, a);
+      hipLaunchKernelGGL((expo_kernel<kInt, kInj, 16, false, false, kTry>), grid, dim3(kBlock), 0, st, a);
   } else if (variant == REPLAY_V_NODE4) {
     if (a.W % 4 == 0 && c8)
-      hipLaunchKernelGGL((expo_kernel<kInt, kInj, 4, true, true>), grid, dim3(kBlock), 0, st, a);
+      hipLaunchKernelGGL((expo_kernel<kInt, kInj, 4, true, true, kTry>), grid, dim3(kBlock), 0, st, a);
     else if (a.W % 4 == 0)
-      hipLaunchKernelGGL((expo_kernel<kInt, kInj, 4, true, false>), grid, dim3(kBlock), 0, st, a);
+      hipLaunchKernelGGL((expo_kernel<kInt, kInj, 4, true, false, kTry>), grid, dim3(kBlock), 0, st, a);
     else
-      hipLaunchKernelGGL((expo_kernel<kInt, kInj, 4, false, false>), grid, dim3(kBlock), 0, st, a);
+      hipLaunchKernelGGL((expo_kernel<kInt, kInj, 4, false, false, kTry>), grid, dim3(kBlock), 0, st, a);
   } else {
-    hipLaunchKernelGGL((expo_kernel<kInt, kInj, 1, false, false>), grid, dim3(kBlock), 0, st, a);
+    hipLaunchKernelGGL((expo_kernel<kInt, kInj, 1, false, false, kTry>), grid, dim3(kBlock), 0, st, a);
   }
+}
+
+// try_int: a float window whose weights are integers (divisor 1), summed in
+// integers first
+template <bool kInt, bool kInj>
+void launch_expo(const StepArgs &a, int variant, hipStream_t st, bool try_int) {
+  if constexpr (!kInt) {
+    if (try_int) {
+      launch_expo_t<false, kInj, true>(a, variant, st);
+      return;
+    }
+  }
+  launch_expo_t<kInt, kInj, false>(a, variant, st);
 }
 
 // Planes the bit-sliced counters need for rows of up to maxrow contacts
@@ -1287,6 +1329,8 @@ void launch_long(const StepArgs &a, bool int_mode, const LongRows &lr, hipStream
   if (int_mode) {
     launch_long_part<true, kInj, false>(a, lr.rows, lr.n_int, st);
     launch_long_part<false, kInj, true>(a, lr.rows + lr.n_int, lr.n_all - lr.n_int, st);
+  } else if (a.scale == 1.0f) {
+    launch_long_part<false, kInj, true>(a, lr.rows, lr.n_all, st);
   } else {
     launch_long_part<false, kInj, false>(a, lr.rows, lr.n_all, st);
   }
@@ -1307,13 +1351,13 @@ void launch_step(const StepArgs &a, int variant, bool int_mode, bool injected, d
     else
       launch_sliced_nb<16>(a, injected, st);
   } else if (int_mode && !injected) {
-    launch_expo<true, false>(a, variant, st);
+    launch_expo<true, false>(a, variant, st, false);
   } else if (int_mode) {
-    launch_expo<true, true>(a, variant, st);
+    launch_expo<true, true>(a, variant, st, false);
   } else if (!injected) {
-    launch_expo<false, false>(a, variant, st);
+    launch_expo<false, false>(a, variant, st, a.scale == 1.0f);
   } else {
-    launch_expo<false, true>(a, variant, st);
+    launch_expo<false, true>(a, variant, st, a.scale == 1.0f);
   }
   if (injected)
     launch_long<true>(a, int_mode, lr, st);

@@ -144,6 +144,49 @@ def test_long_row_sums_beyond_2_24_match_oracle(oracle):
     np.testing.assert_array_equal(state.reshape(-1), ostate)
 
 
+@pytest.mark.parametrize("M", [130, 256])
+@pytest.mark.parametrize("seeded", [0.05, 0.9])
+def test_daily_windows_beyond_2_24_match_oracle(oracle, M, seeded):
+    """Daily windows (step = window = 86400 s) over nodes with about 600
+    multi-day contacts each: every short row's weight bound passes 2^24, so
+    the windows are not integer windows although their weights are
+    integers, and each group is summed in integers first.  With 5% seeded
+    the replicas' sums stay below 2^24 (the integer sums decide); with 90%
+    seeded a susceptible replica's sum passes it (the reference order
+    decides).  A small infection probability keeps P near 0.4."""
+    import ctypes as C
+    from test_host_slices import HostPlan
+    rng = np.random.default_rng(13)
+    n, k = 400, 120000
+    a = rng.integers(0, n, k)
+    b = (a + 1 + rng.integers(0, n - 1, k)) % n
+    t0 = 946713600
+    s = t0 + rng.integers(-2 * 86400, 3 * 86400, k)
+    e = s + rng.integers(4 * 86400, 14 * 86400, k)
+    p = Params(M=M, step_size=86400.0, static_network_duration=86400, iterations=10,
+               initial_infected=seeded, infect_prob=1e-4, exposed_duration=0,
+               infectious_duration=20 * 86400, resistant_duration=10 ** 7, seed=7,
+               start_time=t0)
+    # every window has integer weights but is not an integer window
+    L = replay_amd.load_library()
+    L.rp_host_plan_open.argtypes = [C.c_int32, C.c_int64] + [C.POINTER(C.c_int32)] * 2 + \
+        [C.POINTER(C.c_int64)] * 2 + [C.POINTER(replay_amd._Params), C.POINTER(C.c_void_p)]
+    L.rp_host_plan_int_mode.argtypes = [C.c_void_p, C.c_int32]
+    L.rp_host_plan_int_mode.restype = C.c_int32
+    L.rp_host_plan_close.argtypes = [C.c_void_p]
+    hp = HostPlan(L, n, a, b, s, e, p)
+    assert [L.rp_host_plan_int_mode(hp.h, it) for it in range(p.iterations)] == [0] * 10
+    hp.close()
+    with Simulation.from_contacts(n, a.astype(np.int32), b.astype(np.int32), s, e, p) as sim:
+        sim.run()
+        counts, state = sim.counts(), sim.state()
+    g = oracle.OracleGraph.from_contacts(n, a.astype(np.int32), b.astype(np.int32), s, e)
+    oc, ostate, _ = g.simulate(_to_oracle_params(oracle, p, threads=8), want_state=True)
+    np.testing.assert_array_equal(counts, oc)
+    np.testing.assert_array_equal(state.reshape(-1), ostate)
+    assert (counts[-1, :, 1] + counts[-1, :, 2]).max() > seeded * n  # infections were drawn
+
+
 def test_long_row_exposure_matches_oracle_order(oracle):
     """Exposure of hub rows (integer-capable and beyond 2^24 seconds) equals
     the sequential fp32 sum of the reference CSR bit for bit."""
