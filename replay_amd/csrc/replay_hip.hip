@@ -435,6 +435,8 @@ __device__ __forceinline__ void group_exposure(const StepArgs &a, int e0, int e1
 // the reference's fp32 order is an integer that fp32 holds exactly, so the
 // integer sum is that order's result; the caller falls back to the
 // reference order when a susceptible replica's sum reaches 2^24.
+constexpr int kTryIntMinEntries = 32;
+
 __device__ __forceinline__ bool try_integer_sums(const StepArgs &a, int e0, int e1) {
   const int64_t L = static_cast<int64_t>(a.we_rel) - a.ws_rel;
   return L < (int64_t(1) << 24) && static_cast<int64_t>(e1 - e0) * L < (int64_t(1) << 32);
@@ -574,7 +576,9 @@ void expo_kernel(StepArgs a) {
       float E[WT];
       bool done = false;
       if constexpr (kTryInt) {
-        if (try_integer_sums(a, e0, e1)) {
+        // rows of a few contacts keep the float path, which visits only the
+        // active entries and costs less there than a column count per word
+        if (e1 - e0 > kTryIntMinEntries && try_integer_sums(a, e0, e1)) {
           group_exposure<true, WT, kFull, EnT>(a, e0, e1, rlim, q0, nq, lane, ss, en0, en1, E);
           bool big = false;
 #pragma unroll
