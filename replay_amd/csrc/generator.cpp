@@ -8,6 +8,7 @@
 // C library's additive feedback generator (degree 31, separation 3), seeded
 // as srand() seeds it, so the output does not depend on the C library the
 // program runs on and no global generator state is touched.
+#include <algorithm>
 #include <cstdint>
 #include <cstdio>
 #include <cstring>
@@ -98,14 +99,18 @@ extern "C" int32_t replay_generate_groups(int64_t base_time, int32_t steps, int3
     if (!buf.empty()) std::fwrite(buf.data(), 1, buf.size(), f);
     buf.clear();
   };
-  auto put = [&](const char *s, size_t n) {
-    buf.insert(buf.end(), s, s + n);
+  // n: what snprintf returned for a buffer of cap bytes (clamped to what it
+  // wrote; nothing on an encoding error)
+  auto put = [&](const char *s, int n, size_t cap) {
+    if (n <= 0) return;
+    buf.insert(buf.end(), s, s + std::min(static_cast<size_t>(n), cap - 1));
     if (buf.size() >= (1u << 20)) flush();
   };
   char tmp[96];
-  put("NODE_LIST", 9);
-  for (int32_t i = 1; i <= population; ++i) put(tmp, std::snprintf(tmp, sizeof tmp, " %d", i));
-  put("\n", 1);
+  put("NODE_LIST", 9, 10);
+  for (int32_t i = 1; i <= population; ++i)
+    put(tmp, std::snprintf(tmp, sizeof tmp, " %d", i), sizeof tmp);
+  put("\n", 1, 2);
   LibcRand rng(static_cast<uint32_t>(seed));
   int64_t count = 0;
   char a[21], b[21];
@@ -119,8 +124,10 @@ extern "C" int32_t replay_generate_groups(int64_t base_time, int32_t steps, int3
           iso_utc(t0 + rng.next() % step_size, a);
           iso_utc(t0 + rng.next() % step_size, b);
           const bool swap = std::strcmp(a, b) > 0;
-          put(tmp, std::snprintf(tmp, sizeof tmp, "%d, %d, %s, %s;\n", i + 1, j + 1,
-                                 swap ? b : a, swap ? a : b));
+          put(tmp,
+              std::snprintf(tmp, sizeof tmp, "%d, %d, %s, %s;\n", i + 1, j + 1, swap ? b : a,
+                            swap ? a : b),
+              sizeof tmp);
           ++count;
         }
       }
