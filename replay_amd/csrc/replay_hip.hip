@@ -216,6 +216,15 @@ __device__ __forceinline__ uint32_t column_count(uint64_t x, int lane) {
   return __builtin_amdgcn_ubfe(u, (lane & 32) ? 16u : 0u, 16u);
 }
 
+// Lane j's 64-bit word, broadcast to the wavefront (wave-uniform).
+__device__ __forceinline__ uint64_t lane_word(uint64_t m, int j) {
+  const uint32_t lo = static_cast<uint32_t>(
+      __builtin_amdgcn_readlane(static_cast<int>(static_cast<uint32_t>(m)), j));
+  const uint32_t hi = static_cast<uint32_t>(
+      __builtin_amdgcn_readlane(static_cast<int>(static_cast<uint32_t>(m >> 32)), j));
+  return (static_cast<uint64_t>(hi) << 32) | lo;
+}
+
 typedef int v4i __attribute__((ext_vector_type(4)));
 
 // Contact entries e .. e+63 of a row that ends at e1 (e < e1), one per lane;
@@ -342,15 +351,12 @@ __device__ __forceinline__ void group_exposure(const StepArgs &a, int e0, int e1
         // few contributing contacts: add them one at a time
         const bool few = __popcll(nz & ~partial) <= kFewContacts;
         const uint32_t c = (!few && (nz & ~partial)) ? column_count(full ? m : 0ull, lane) : 0u;
+        // the word of contact j is broadcast and read as a lane mask
+        // (inverse ballot): replica lane r tests bit r in one select
         for (uint64_t pm = few ? nz : (partial & nz); pm != 0ull; pm &= pm - 1) {
           const int j = __builtin_ctzll(pm);
-          const uint32_t lo = static_cast<uint32_t>(
-              __builtin_amdgcn_readlane(static_cast<int>(static_cast<uint32_t>(m)), j));
-          const uint32_t hi = static_cast<uint32_t>(
-              __builtin_amdgcn_readlane(static_cast<int>(static_cast<uint32_t>(m >> 32)), j));
-          const uint64_t mj = (static_cast<uint64_t>(hi) << 32) | lo;
           const uint32_t ovj = static_cast<uint32_t>(__builtin_amdgcn_readlane(ov, j));
-          iacc[q] += static_cast<uint32_t>((mj >> lane) & 1ull) * ovj;
+          iacc[q] += __builtin_amdgcn_inverse_ballot_w64(lane_word(m, j)) ? ovj : 0u;
         }
         iacc[q] += static_cast<uint32_t>(L) * c;
       }
@@ -366,10 +372,10 @@ __device__ __forceinline__ void group_exposure(const StepArgs &a, int e0, int e1
         if (colj != cur_col) {  // flush the finished column
           if (cur_any) {
 #pragma unroll
-            for (int q = 0; q < WT; ++q) {
-              const uint32_t word = lane < 32 ? cur_lo[q] : cur_hi[q];
-              if ((word >> (lane & 31)) & 1u) facc[q] += cur_w;
-            }
+            for (int q = 0; q < WT; ++q)
+              if (__builtin_amdgcn_inverse_ballot_w64((static_cast<uint64_t>(cur_hi[q]) << 32) |
+                                                      cur_lo[q]))
+                facc[q] += cur_w;
           }
           cur_col = colj;
           cur_w = 0.0f;
@@ -419,10 +425,9 @@ __device__ __forceinline__ void group_exposure(const StepArgs &a, int e0, int e1
   }
   if (!kIntMode && cur_any) {
 #pragma unroll
-    for (int q = 0; q < WT; ++q) {
-      const uint32_t word = lane < 32 ? cur_lo[q] : cur_hi[q];
-      if ((word >> (lane & 31)) & 1u) facc[q] += cur_w;
-    }
+    for (int q = 0; q < WT; ++q)
+      if (__builtin_amdgcn_inverse_ballot_w64((static_cast<uint64_t>(cur_hi[q]) << 32) | cur_lo[q]))
+        facc[q] += cur_w;
   }
 #pragma unroll
   for (int q = 0; q < WT; ++q) E[q] = kIntMode ? static_cast<float>(iacc[q]) : facc[q];
