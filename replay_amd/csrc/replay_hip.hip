@@ -457,7 +457,8 @@ __device__ __forceinline__ uint64_t draw_word(const StepArgs &a, const NetDev &n
                                               int ii, int Q, int lane, uint64_t ss, float E,
                                               uint32_t (&pw)[4], int &pw_grp) {
   const int r = Q * kWave + lane;
-  const bool sus = (ss >> lane) & 1ull;
+  // ss is wave-uniform: read as a lane mask, replica lane r tests bit r
+  const bool sus = __builtin_amdgcn_inverse_ballot_w64(ss);
   // E == 0 gives P = 0 (or NaN), never above a uniform in [0, 1)
   const bool need = sus && (kInjected || E > 0.0f);
   // The block serving words 4g .. 4g+3 is computed once per wavefront, by
@@ -587,7 +588,8 @@ void expo_kernel(StepArgs a) {
           group_exposure<true, WT, kFull, EnT>(a, e0, e1, rlim, q0, nq, lane, ss, en0, en1, E);
           bool big = false;
 #pragma unroll
-          for (int q = 0; q < WT; ++q) big |= ((ss[q] >> lane) & 1ull) && E[q] >= 16777216.0f;
+          for (int q = 0; q < WT; ++q)
+            big |= __builtin_amdgcn_inverse_ballot_w64(ss[q]) && E[q] >= 16777216.0f;
           done = __ballot(big) == 0ull;
         }
       }
@@ -647,7 +649,7 @@ __global__ __launch_bounds__(kBlock) void long_row_kernel(StepArgs a, const int3
       group_exposure<true, 1, false, v4i>(a, e0, e1, rlim, q, 1, lane, ss, en0, en1, E);
       // E is the integer sum converted to float: >= 2^24 exactly when the
       // sum is
-      const bool sus = (ss[0] >> lane) & 1ull;
+      const bool sus = __builtin_amdgcn_inverse_ballot_w64(ss[0]);
       done = __ballot(sus && E[0] >= 16777216.0f) == 0ull;
     }
     if (!done)
