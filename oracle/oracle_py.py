@@ -165,7 +165,9 @@ class OracleGraph:
             raise ValueError(f"NCLS build failed: {rc}")
 
     def __del__(self):
-        if getattr(self, "ptr", None):
+        # at interpreter exit the module's globals may already be cleared:
+        # the process releases the graph then
+        if getattr(self, "ptr", None) and callable(lib):
             lib().or_graph_free(self.ptr)
             self.ptr = None
 
