@@ -180,6 +180,42 @@ def test_every_variant_matches_oracle_wide(oracle, M, step, D, flags):
             np.testing.assert_array_equal(sim.state(), state, err_msg=f"variant {v}")
 
 
+SWEEP_WINDOWS = [(600.0, 600), (600.0, 1800), (3600.0, 3600), (1000.0, 3600), (450.0, 3600),
+                 (300.0, 600)]
+
+
+@pytest.mark.parametrize("case", range(48))
+def test_random_sweep_every_variant(oracle, case):
+    """Random graphs of 12-160 nodes and 100-6000 contacts, 1-1024 replicas
+    (one word to 16 whole words, ragged last words), integer and float
+    windows, both entry widths, 0-60% long contacts and low to high infection
+    probabilities: every exposure variant's counts and states equal the
+    oracle's.  Each case draws its parameters from its own seed."""
+    rng = np.random.default_rng(1000 + case)
+    seed = int(rng.integers(0, 1 << 30))
+    n = int(rng.choice([12, 40, 90, 160]))
+    e = int(rng.choice([100, 600, 2500, 6000]))
+    M = int(rng.choice([1, 33, 64, 130, 256, 1024]))
+    step, D = SWEEP_WINDOWS[int(rng.integers(0, len(SWEEP_WINDOWS)))]
+    flags = int(rng.choice([0, replay_amd.F_WIDE_ENTRIES]))
+    long_frac = float(rng.choice([0.0, 0.2, 0.6]))
+    print(f"case {case}: seed={seed} n={n} e={e} M={M} step={step} D={D} flags={flags}", flush=True)
+    nn, a, b, s, en = _random_case(seed, n=n, e=e, long_frac=long_frac)
+    p = Params(M=M, step_size=step, static_network_duration=D, iterations=40,
+               initial_infected=0.1, infect_prob=float(rng.choice([0.05, 0.3, 0.9])),
+               exposed_duration=1200, infectious_duration=5400, resistant_duration=3600,
+               seed=seed % 1000, flags=flags)
+    g = oracle.OracleGraph.from_contacts(nn, a, b, s, en)
+    counts, state, _ = g.simulate(_to_oracle_params(oracle, p, threads=8), want_state=True)
+    with Simulation.from_contacts(nn, a, b, s, en, p) as sim:
+        for v in (replay_amd.V_WORD, replay_amd.V_NODE4, replay_amd.V_NODE16, replay_amd.V_SLICED):
+            sim.set_variant(v)
+            sim.reset()
+            sim.run()
+            np.testing.assert_array_equal(sim.counts(), counts, err_msg=f"variant {v}")
+            np.testing.assert_array_equal(sim.state(), state, err_msg=f"variant {v}")
+
+
 @pytest.mark.parametrize("M", [70, 256])
 def test_injected_uniform_stream(oracle, M):
     n, a, b, s, e = _random_case(3, n=50, e=1200)
