@@ -190,7 +190,9 @@ def test_random_sweep_every_variant(oracle, case):
     (one word to 16 whole words, ragged last words), integer and float
     windows, both entry widths, 0-60% long contacts and low to high infection
     probabilities: every exposure variant's counts and states equal the
-    oracle's.  Each case draws its parameters from its own seed."""
+    oracle's.  Each case draws its parameters from its own seed.  Beside
+    test_gpu_fuzz (one forced variant per case, at most 5 words), this runs
+    all four variants on each graph and reaches the whole 16-word groups."""
     rng = np.random.default_rng(1000 + case)
     seed = int(rng.integers(0, 1 << 30))
     n = int(rng.choice([12, 40, 90, 160]))
