@@ -117,3 +117,38 @@ def test_config3_reduced_batch_vs_plain():
             s.run()
             np.testing.assert_array_equal(s.counts(), bc[:, k])
     assert bc[-1, :, :, 1:3].sum() > 0
+
+
+@pytest.mark.parametrize("case", range(16))
+def test_random_batches_every_variant(oracle, case):
+    """Random batches of 2-6 uneven networks (5-150 nodes, their own
+    infection probability, durations and seeding), 33-1024 replicas, integer
+    and float windows: under every exposure variant, each network's counts
+    equal the oracle's for that network alone with its stream id, and the
+    batch's states are the same under every variant."""
+    rng = np.random.default_rng(7000 + case)
+    sizes = [int(x) for x in rng.integers(5, 151, int(rng.integers(2, 7)))]
+    nets = _nets(int(rng.integers(0, 1 << 20)), sizes, long_frac=float(rng.choice([0.0, 0.3, 0.6])))
+    step, D = [(600.0, 600), (600.0, 1800), (1000.0, 3600), (3600.0, 3600)][int(rng.integers(0, 4))]
+    M = int(rng.choice([33, 64, 256, 1024]))
+    shared = Params(M=M, step_size=step, static_network_duration=D, iterations=50,
+                    seed=int(rng.integers(0, 1 << 30)))
+    print(f"case {case}: sizes={sizes} M={M} step={step} D={D}", flush=True)
+    expected = []
+    for nw in nets:
+        g = oracle.OracleGraph.from_contacts(nw.n_nodes, nw.src, nw.tgt, nw.start, nw.end)
+        op = _to_oracle_params(oracle, _plain_params(shared, nw))
+        op.stream = nw.stream
+        expected.append(g.simulate(op)[0])
+    states = []
+    with Simulation.from_networks(nets, shared) as sim:
+        for v in (replay_amd.V_WORD, replay_amd.V_NODE4, replay_amd.V_NODE16, replay_amd.V_SLICED):
+            sim.set_variant(v)
+            sim.reset()
+            sim.run()
+            bc = sim.counts()
+            for k, counts in enumerate(expected):
+                np.testing.assert_array_equal(bc[:, k], counts, err_msg=f"variant {v} network {k}")
+            states.append(sim.state())
+    for s in states[1:]:
+        np.testing.assert_array_equal(s, states[0])
