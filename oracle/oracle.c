@@ -880,6 +880,15 @@ void or_spmm_f64(const int32_t *rowptr, const int32_t *colind, const float *val,
     }
 }
 
+/* Seconds of contact per unit of infect_prob: the reference normalises the
+   exposure to hours (cu:127-128, 449-450), whatever the step size.  A
+   compile-time macro only so that tests/test_oracle_probability.py can build
+   a copy with the reference's commented-out E / step_size (cu:449) and show
+   that its closed-form checks reject it. */
+#ifndef OR_HOUR_DIVISOR
+#define OR_HOUR_DIVISOR(p) 3600.0f
+#endif
+
 /* cu:439-458 */
 void or_infection_probability(const or_params *p, int n_nodes, const int32_t *c,
                               float *E) {
@@ -887,7 +896,7 @@ void or_infection_probability(const or_params *p, int n_nodes, const int32_t *c,
   size_t total = (size_t)n_nodes * (size_t)p->M;
   for (size_t idx = 0; idx < total; idx++) {
     if (c[idx] == 0) {
-      float e = E[idx] / 3600.0f;
+      float e = E[idx] / OR_HOUR_DIVISOR(p);
       float x = log_base * e;
       E[idx] = 1.0f - (p->use_libm_exp ? expf(x) : or_expf(x));
     } else {

@@ -98,6 +98,8 @@ def lib() -> C.CDLL:
         L.or_spmm_f64.argtypes = [P(C.c_int32), P(C.c_int32), P(C.c_float), C.c_int,
                                   P(C.c_float), P(C.c_double), C.c_int]
         L.or_state_class.argtypes = [P(Params), C.c_int32]
+        L.or_infection_probability.argtypes = [P(Params), C.c_int, P(C.c_int32), P(C.c_float)]
+        L.or_infection_probability.restype = None
         L.or_window.argtypes = [P(Params), C.c_void_p, C.c_int, P(C.c_int64), P(C.c_int64),
                                 P(C.c_int64), P(C.c_float)]
         _lib = L
@@ -250,6 +252,19 @@ def uniform(params: Params, slot: int, node: int, replica: int, n_nodes: int) ->
 
 def state_class(params: Params, c: int) -> int:
     return lib().or_state_class(C.byref(params), c)
+
+
+def infection_probability(params: Params, c, E, library: C.CDLL | None = None) -> np.ndarray:
+    """P per (node, replica) from countdowns c and exposure seconds E, both
+    N x M column-major (cu:439-458), by the oracle (or by `library`, a build
+    of oracle.c loaded elsewhere, with the same signature)."""
+    L = library if library is not None else lib()
+    c = np.ascontiguousarray(c, dtype=np.int32)
+    out = np.array(E, dtype=np.float32, copy=True)
+    assert c.size == out.size and c.size % params.M == 0
+    L.or_infection_probability(C.byref(params), c.size // params.M, _ptr(c, C.c_int32),
+                               _ptr(out, C.c_float))
+    return out
 
 
 # ---------------------------------------------------------------------------
