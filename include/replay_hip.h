@@ -26,7 +26,7 @@
 extern "C" {
 #endif
 
-#define REPLAY_HIP_ABI_VERSION 2
+#define REPLAY_HIP_ABI_VERSION 3
 
 enum {
   REPLAY_OK = 0,
@@ -111,6 +111,9 @@ typedef struct replay_info {
   int32_t graph_ready;        /* a captured full-run hipGraph exists */
   int32_t entry_bytes;        /* bytes per packed entry the step kernels stream:
                                  8 (the 8-byte copy) or 16 */
+  float tune_ms[4];           /* device ms of each candidate in the last replay_autotune,
+                                 in the order WORD, NODE4, NODE16, SLICED; -1 where the
+                                 variant does not apply or no autotune ran */
 } replay_info;
 
 /* replay_info.variant */
@@ -161,8 +164,11 @@ int32_t replay_get_network_offsets(replay_ctx *ctx, int32_t *node_off,
                                    int32_t n);
 
 /* Times `steps` iterations of each applicable step-kernel variant (from a
-   reset state, resetting again afterwards), keeps the fastest and returns
-   its REPLAY_V_* code in *variant.  No counterpart in the reference: the
+   reset state, after an untimed warm-up, with eager launches even when
+   REPLAY_F_GRAPH is set; resetting again afterwards), keeps the fastest and
+   returns its REPLAY_V_* code in *variant; replay_info.tune_ms holds the
+   times.  A captured graph is dropped (it is recaptured with the chosen
+   variant by the next full run).  No counterpart in the reference: the
    tile/schedule choice cuSPARSE makes internally (cu:1039-1048). */
 int32_t replay_autotune(replay_ctx *ctx, int32_t steps, int32_t *variant);
 

@@ -23,7 +23,7 @@ CLI_PATH = os.path.join(PKG_DIR, "bin", "replay")
 GEN_PATH = os.path.join(PKG_DIR, "bin", "replay_gen")
 CGG_PATH = os.path.join(PKG_DIR, "bin", "replay_cgg")
 
-ABI_VERSION = 2
+ABI_VERSION = 3
 
 # error codes (include/replay_hip.h)
 E_ARG, E_IO, E_FORMAT, E_NODE_ID, E_DUP_ID, E_NO_EDGES, E_COUNT, E_NODE_RANGE = \
@@ -65,7 +65,8 @@ class _Info(C.Structure):
                 ("state_bytes", C.c_int64), ("steps_done", C.c_int32),
                 ("truncated_windows", C.c_int32), ("step_bytes", C.c_int64),
                 ("ring_depth", C.c_int32), ("n_networks", C.c_int32), ("variant", C.c_int32),
-                ("graph_ready", C.c_int32), ("entry_bytes", C.c_int32)]
+                ("graph_ready", C.c_int32), ("entry_bytes", C.c_int32),
+                ("tune_ms", C.c_float * 4)]
 
 
 F_GRAPH = 1
@@ -312,7 +313,9 @@ class Simulation:
     def info(self) -> dict:
         inf = _Info()
         self._check(load_library().replay_get_info(self._ctx, C.byref(inf)))
-        return {name: getattr(inf, name) for name, _ in _Info._fields_}
+        out = {name: getattr(inf, name) for name, _ in _Info._fields_}
+        out["tune_ms"] = list(inf.tune_ms)
+        return out
 
     def reset(self) -> None:
         self._check(load_library().replay_reset(self._ctx))

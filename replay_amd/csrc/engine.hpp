@@ -111,6 +111,7 @@ struct replay_ctx {
   int truncated = 0;
   int64_t n_contacts = 0, nlists = 0;
   int32_t variant = REPLAY_V_WORD;
+  float tune_ms[4] = {-1.0f, -1.0f, -1.0f, -1.0f};  // last autotune: WORD, NODE4, NODE16, SLICED
   hipStream_t stream = nullptr;
   hipGraphExec_t graph = nullptr;
   // device

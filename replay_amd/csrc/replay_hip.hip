@@ -1959,6 +1959,7 @@ int32_t replay_get_info(replay_ctx *ctx, replay_info *out) {
   out->variant = ctx->variant;
   out->graph_ready = ctx->graph != nullptr;
   out->entry_bytes = c8 ? 8 : 16;
+  for (int v = 0; v < 4; ++v) out->tune_ms[v] = ctx->tune_ms[v];
   return REPLAY_OK;
 }
 
@@ -2014,27 +2015,43 @@ int32_t replay_autotune(replay_ctx *ctx, int32_t steps, int32_t *variant) {
   if (!ctx || !variant) return REPLAY_E_ARG;
   steps = std::min(steps, ctx->T);
   HIP_OR_FAIL(ctx, hipSetDevice(ctx->p.device));
+  // A graph captured with one variant would be replayed for every candidate
+  // of a full run: drop it, and time eager launches (the flag is restored
+  // on every exit; the next full run recaptures with the chosen variant).
+  HIP_OR_FAIL(ctx, hipStreamSynchronize(ctx->stream));
+  if (ctx->graph) {
+    (void)hipGraphExecDestroy(ctx->graph);
+    ctx->graph = nullptr;
+  }
+  struct FlagRestore {
+    replay_ctx *c;
+    int32_t flags;
+    ~FlagRestore() { c->p.flags = flags; }
+  } restore{ctx, ctx->p.flags};
+  ctx->p.flags &= ~REPLAY_F_GRAPH;
   const int candidates[] = {REPLAY_V_WORD, REPLAY_V_NODE4, REPLAY_V_NODE16, REPLAY_V_SLICED};
   int best = ctx->variant;
   double best_ms = 1e30;
-  for (int v : candidates) {
+  for (int c = 0; c < 4; ++c) {
+    const int v = candidates[c];
+    ctx->tune_ms[c] = -1.0f;
     if (!variant_applies(ctx, v) || steps <= 0) continue;
     ctx->variant = v;
+    // untimed warm-up: the first launch of a kernel also loads its code
     int rc = do_reset(ctx);
+    if (rc == REPLAY_OK) rc = replay_run(ctx, std::min<int32_t>(steps, 2));
+    if (rc == REPLAY_OK) rc = do_reset(ctx);
     if (rc != REPLAY_OK) return rc;
     double ms = 0.0;
     rc = replay_run_timed(ctx, steps, &ms);
     if (rc != REPLAY_OK) return rc;
+    ctx->tune_ms[c] = static_cast<float>(ms);
     if (ms < best_ms) {
       best_ms = ms;
       best = v;
     }
   }
   ctx->variant = best;
-  if (ctx->graph) {  // captured with another variant
-    hipGraphExecDestroy(ctx->graph);
-    ctx->graph = nullptr;
-  }
   *variant = best;
   return do_reset(ctx);
 }

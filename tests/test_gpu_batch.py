@@ -105,6 +105,36 @@ def test_variants_agree(M):
         np.testing.assert_array_equal(s, results[0][1])
 
 
+def test_autotune_times_every_variant_under_graph_flag():
+    """Open-time autotune with REPLAY_F_GRAPH and T <= 32 (the autotune
+    covers the whole run): every applicable variant is timed by eager
+    launches, no graph of a candidate survives, and the first full run after
+    it replays a graph of the chosen variant with the eager results."""
+    nets = _nets(11, [60, 90, 33])
+    shared = Params(M=256, step_size=900.0, static_network_duration=1800, iterations=24, seed=3)
+    with Simulation.from_networks(nets, shared) as eager:
+        eager.run()
+        ref = eager.counts()
+    gp = Params(**{**shared.__dict__, "flags": replay_amd.F_GRAPH | replay_amd.F_AUTOTUNE})
+    with Simulation.from_networks(nets, gp) as g:
+        info = g.info()
+        assert info["graph_ready"] == 0
+        assert all(t > 0 for t in info["tune_ms"]), info["tune_ms"]  # W=4: all four apply
+        g.run()
+        assert g.info()["graph_ready"] == 1
+        np.testing.assert_array_equal(g.counts(), ref)
+        first = g.info()["variant"]
+        g.autotune(24)  # drops the graph; the next full run captures again
+        info = g.info()
+        assert info["graph_ready"] == 0 and all(t > 0 for t in info["tune_ms"])
+        g.reset()
+        g.run()
+        assert g.info()["graph_ready"] == 1
+        np.testing.assert_array_equal(g.counts(), ref)
+        assert first in (replay_amd.V_WORD, replay_amd.V_NODE4, replay_amd.V_NODE16,
+                         replay_amd.V_SLICED)
+
+
 def test_config3_reduced_batch_vs_plain():
     w = workloads.config3_batched(n_networks=6, n_min=300, n_max=3000, iterations=48,
                                   mean_days=2.0)
