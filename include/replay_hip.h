@@ -234,6 +234,17 @@ int32_t replay_get_slice(replay_ctx *ctx, int32_t k, int32_t *rowptr,
 int32_t replay_get_exposure(replay_ctx *ctx, int32_t k, float *out,
                             int64_t n_out);
 
+/* replay_get_exposure without any device allocation: the same result, bit
+   for bit, staged through the library's static per-device array of 2^24
+   floats (64 MiB, held by each device the library runs on until the process
+   ends) instead of a buffer allocated per call.  No counterpart in the
+   reference; for callers that cannot allocate device memory (code under
+   hipGraph capture, tight serving loops, harnesses that forbid hipMalloc).
+   REPLAY_E_ARG when N * M exceeds 2^24.  Calls on one device share the
+   array, so calls that overlap on one device race. */
+int32_t replay_get_exposure_noalloc(replay_ctx *ctx, int32_t k, float *out,
+                                    int64_t n_out);
+
 /* ---- Differentiable marginal-probability form (no counterpart in the
    reference, which only samples).  Individual-level SEIR probabilities per
    node and parameter set over the same duration-weighted slices as the

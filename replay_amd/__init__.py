@@ -84,6 +84,7 @@ EXPORTED_SYMBOLS = (
     "replay_open_batch", "replay_get_network_offsets", "replay_autotune", "replay_set_variant",
     "replay_marginal_create", "replay_marginal_forward", "replay_marginal_backward",
     "replay_marginal_destroy", "replay_generate_groups", "replay_contactgraph",
+    "replay_get_exposure_noalloc",
 )
 
 _lib = None
@@ -139,6 +140,7 @@ def load_library() -> C.CDLL:
     L.replay_get_slice.argtypes = [ctx_p, C.c_int32, P(C.c_int32), P(C.c_int32), P(C.c_float),
                                    C.c_int64, P(C.c_int64)]
     L.replay_get_exposure.argtypes = [ctx_p, C.c_int32, P(C.c_float), C.c_int64]
+    L.replay_get_exposure_noalloc.argtypes = [ctx_p, C.c_int32, P(C.c_float), C.c_int64]
     L.replay_last_error.argtypes = [ctx_p]
     L.replay_last_error.restype = C.c_char_p
     L.replay_error_string.argtypes = [C.c_int32]
@@ -162,6 +164,7 @@ def load_library() -> C.CDLL:
     for name in ("replay_open", "replay_open_contacts", "replay_get_info", "replay_reset",
                  "replay_run", "replay_run_timed", "replay_sync", "replay_get_counts", "replay_get_state",
                  "replay_set_uniforms", "replay_get_slice", "replay_get_exposure",
+                 "replay_get_exposure_noalloc",
                  "replay_open_batch", "replay_get_network_offsets", "replay_autotune",
                  "replay_set_variant"):
         getattr(L, name).restype = C.c_int32
@@ -397,11 +400,14 @@ class Simulation:
                                                     None, -1, C.byref(nnz)))
         return nnz.value
 
-    def exposure(self, k: int) -> np.ndarray:
-        """Exposure seconds (N*M, column-major) of the current state in window k."""
+    def exposure(self, k: int, noalloc: bool = False) -> np.ndarray:
+        """Exposure seconds (N*M, column-major) of the current state in window
+        k; noalloc stages it in the library's static device array instead of
+        a buffer allocated for the call (N*M up to 2^24)."""
         out = np.zeros(self.n_nodes * self.M, np.float32)
-        self._check(load_library().replay_get_exposure(self._ctx, k, _ptr(out, C.c_float),
-                                                       out.size))
+        L = load_library()
+        fn = L.replay_get_exposure_noalloc if noalloc else L.replay_get_exposure
+        self._check(fn(self._ctx, k, _ptr(out, C.c_float), out.size))
         return out
 
 

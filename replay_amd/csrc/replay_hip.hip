@@ -42,6 +42,7 @@
 
 #include <algorithm>
 #include <atomic>
+#include <mutex>
 #include <climits>
 #include <cmath>
 #include <cstdio>
@@ -56,6 +57,13 @@
 namespace rp {
 thread_local std::string g_open_error;
 }  // namespace rp
+
+// Staging of replay_get_exposure_noalloc: the exposure matrix of one call,
+// written by exposure_kernel and copied to the caller's host buffer.  One
+// array per device, held until the process ends; its device address comes
+// from hipGetSymbolAddress (in host code the name is only a stand-in).
+constexpr int64_t kNoAllocFloats = int64_t(1) << 24;  // 64 MiB
+static __device__ __align__(256) float scratch[kNoAllocFloats];
 
 using namespace rp;
 
@@ -1789,6 +1797,28 @@ NetHost single_net(const replay_params *p) {
   return nh;
 }
 
+// Device address of `scratch` on the context's device (current when
+// called): looked up once per device with hipGetSymbolAddress and kept;
+// later calls on that device reuse it.  Nothing is allocated.
+int noalloc_scratch(replay_ctx *ctx, float **ptr) {
+  static std::mutex mu;
+  static std::vector<float *> addr;  // per device ordinal, null until looked up
+  std::lock_guard<std::mutex> lock(mu);
+  const int dev = ctx->p.device;
+  if (dev < 0) return fail(ctx, REPLAY_E_ARG, "invalid device ordinal");
+  if (static_cast<size_t>(dev) >= addr.size()) addr.resize(static_cast<size_t>(dev) + 1, nullptr);
+  if (!addr[dev]) {
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      return fail(ctx, REPLAY_E_DEVICE,
+                  std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    addr[dev] = static_cast<float *>(p);
+  }
+  *ptr = addr[dev];
+  return REPLAY_OK;
+}
+
 // Captures the step and counts kernels of a full run (iterations 0..T-1
 // from a reset state) into one executable graph.
 int build_graph(replay_ctx *ctx) {
@@ -2197,6 +2227,33 @@ int32_t replay_get_exposure(replay_ctx *ctx, int32_t k, float *out, int64_t n_ou
   if (rc) return rc;
   DeviceScratch scratch{{d_out}};
   dim3 grid((ctx->N + kWavesPerBlock - 1) / kWavesPerBlock, ctx->W);
+  if (ctx->win_int[ctx->iter_window[k]])
+    hipLaunchKernelGGL(exposure_kernel<true>, grid, dim3(kBlock), 0, ctx->stream, a, d_out);
+  else
+    hipLaunchKernelGGL(exposure_kernel<false>, grid, dim3(kBlock), 0, ctx->stream, a, d_out);
+  HIP_OR_FAIL(ctx, hipGetLastError());
+  HIP_OR_FAIL(ctx, hipMemcpyAsync(out, d_out, static_cast<size_t>(n) * sizeof(float),
+                                  hipMemcpyDeviceToHost, ctx->stream));
+  HIP_OR_FAIL(ctx, hipStreamSynchronize(ctx->stream));
+  return REPLAY_OK;
+}
+
+int32_t replay_get_exposure_noalloc(replay_ctx *ctx, int32_t k, float *out, int64_t n_out) {
+  if (!ctx || !out || k < 0 || k >= ctx->T) return REPLAY_E_ARG;
+  const int64_t n = static_cast<int64_t>(ctx->N) * ctx->M;
+  if (n_out < n) return fail(ctx, REPLAY_E_ARG, "exposure buffer too small");
+  if (n > kNoAllocFloats)
+    return fail(ctx, REPLAY_E_ARG,
+                "N * M = " + std::to_string(n) + " exceeds the no-allocation staging limit of " +
+                    std::to_string(kNoAllocFloats) + " floats (64 MiB); use replay_get_exposure");
+  HIP_OR_FAIL(ctx, hipSetDevice(ctx->p.device));
+  float *d_out = nullptr;
+  const int rc = noalloc_scratch(ctx, &d_out);
+  if (rc != REPLAY_OK) return rc;
+  StepArgs a = step_args(ctx, k);
+  dim3 grid((ctx->N + kWavesPerBlock - 1) / kWavesPerBlock, ctx->W);
+  // every (node, sim < M) element is written: nothing of an earlier call's
+  // contents is read back
   if (ctx->win_int[ctx->iter_window[k]])
     hipLaunchKernelGGL(exposure_kernel<true>, grid, dim3(kBlock), 0, ctx->stream, a, d_out);
   else
