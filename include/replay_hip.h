@@ -73,7 +73,8 @@ typedef struct replay_params {
                                    their 8-byte form fits (comparisons and measurement) */
 
 /* One network of a batch (replay_open_batch): its contacts, as in
-   replay_open_contacts, and its own epidemiological parameters. */
+   replay_open_contacts, its own epidemiological parameters and its own
+   number of replicas. */
 typedef struct replay_network {
   int32_t n_node_list;
   const int64_t *node_ids;    /* NULL: ids 0..n_node_list-1 */
@@ -88,11 +89,12 @@ typedef struct replay_network {
   int32_t infectious_duration;
   int32_t resistant_duration;
   uint32_t stream;            /* Philox stream id; equal ids give equal draws */
+  int32_t M;                  /* replicas of this network; 0: replay_params.M */
 } replay_network;
 
 typedef struct replay_info {
   int32_t n_nodes;            /* N (NODE_LIST count; all networks of a batch) */
-  int32_t M;
+  int32_t M;                  /* replicas (a batch: the most of any network) */
   int64_t n_contacts;         /* valid contacts (intervals) */
   int64_t nlists;             /* hit cap of every window query (cu:988-989) */
   int32_t iterations;
@@ -111,6 +113,7 @@ typedef struct replay_info {
   int32_t graph_ready;        /* a captured full-run hipGraph exists */
   int32_t entry_bytes;        /* bytes per packed entry the step kernels stream:
                                  8 (the 8-byte copy) or 16 */
+  int32_t replicas_total;     /* replica columns of all networks (the sum of their M) */
   float tune_ms[4];           /* device ms of each candidate in the last replay_autotune,
                                  in the order WORD, NODE4, NODE16, SLICED; -1 where the
                                  variant does not apply or no autotune ran */
@@ -150,18 +153,26 @@ int32_t replay_open_contacts(int32_t n_node_list, const int64_t *node_ids,
                              replay_ctx **out);
 
 /* Several networks in one context: one step launch advances all of them
-   (the batched form; the reference runs one network per process).  Nodes
-   of network n are rows node_off[n] .. node_off[n+1]-1 of the context; the
-   shared parameters (M, step, iterations, windows, start time, seed, device,
-   flags) come from p, the epidemiological ones from each replay_network.
-   A network gets the same draws as in a single-network context with
-   p->stream equal to its stream id, so both forms give identical results. */
+   (the batched form; the reference runs one network per process, with one
+   global replica count, cu:25, 647-648).  Nodes of network n are rows
+   node_off[n] .. node_off[n+1]-1 of the context and its replicas are the
+   replica columns rep_off[n] .. rep_off[n+1]-1 (replay_get_replica_offsets);
+   the shared parameters (step, iterations, windows, start time, seed,
+   device, flags) come from p, the epidemiological ones and the replica count
+   from each replay_network (M = 0 takes p->M).  A network gets the same draws
+   as in a single-network context with p->stream equal to its stream id and
+   p->M its replica count, so both forms give identical results. */
 int32_t replay_open_batch(int32_t n_networks, const replay_network *nets,
                           const replay_params *p, replay_ctx **out);
 
 /* node_off[0..n_networks]: first row of each network, then the total. */
 int32_t replay_get_network_offsets(replay_ctx *ctx, int32_t *node_off,
                                    int32_t n);
+
+/* rep_off[0..n_networks]: first replica column of each network (its sims
+   in replay_get_counts), then the total.  No counterpart in the reference,
+   which has one network per process. */
+int32_t replay_get_replica_offsets(replay_ctx *ctx, int32_t *rep_off, int32_t n);
 
 /* Times `steps` iterations of each applicable step-kernel variant (from a
    reset state, after an untimed warm-up, with eager launches even when
@@ -200,14 +211,21 @@ int32_t replay_sync(replay_ctx *ctx);
 
 /* Per-iteration SEIR counts in CSV column order (exposed, infectious,
    resistant, susceptible) -- the reduction write_simulation_state performs
-   (cu:505-568).  out holds (steps_done + 1) * n_networks * M * 4 int32,
-   ordered [iteration][network][sim][class]; block 0 is the initial state.
-   Synchronises. */
+   (cu:505-568).  out holds (steps_done + 1) * replicas_total * 4 int32,
+   ordered [iteration][replica column][class], i.e. [iteration][network][sim]
+   [class] with network n's M_n sims from column rep_off[n]; block 0 is the
+   initial state.  Synchronises. */
 int32_t replay_get_counts(replay_ctx *ctx, int32_t *out, int64_t n_out);
 
 /* Countdown state, N x M int32 column-major (cu:1087-1088); for a batch,
-   N counts the rows of all networks.  Synchronises. */
+   N counts the rows of all networks (REPLAY_E_UNSUPPORTED when its networks
+   have different replica counts).  Synchronises. */
 int32_t replay_get_state(replay_ctx *ctx, int32_t *out, int64_t n_out);
+
+/* Countdown state of network `net` alone, N_net x M_net int32 column-major
+   (node - node_off[net] + sim * N_net).  Synchronises. */
+int32_t replay_get_network_state(replay_ctx *ctx, int32_t net, int32_t *out,
+                                 int64_t n_out);
 
 /* Timestamp written with iteration k's block (cu:876, 1093-1095); k = -1
    gives the initial block's time. */
@@ -216,7 +234,8 @@ int64_t replay_iteration_time(replay_ctx *ctx, int32_t k);
 /* Inject a uniform stream for parity tests: u[(slot * M + sim) * N + node],
    slot 0 = initial draw, slot k+1 = iteration k.  n must be
    (iterations + 1) * M * N.  NULL restores Philox.  Redraws the initial
-   state. */
+   state.  REPLAY_E_UNSUPPORTED for a batch whose networks have different
+   replica counts. */
 int32_t replay_set_uniforms(replay_ctx *ctx, const float *u, int64_t n);
 
 /* CSR of iteration k's network window as build_csr_from_intervals
@@ -230,7 +249,9 @@ int32_t replay_get_slice(replay_ctx *ctx, int32_t k, int32_t *rowptr,
 
 /* Exposure seconds E = A_k * I of the current state for iteration k's
    window, as cusparseSpMM (cu:1062-1064) produces it before
-   compute_infection_probability: out[node + sim * N], N * M floats. */
+   compute_infection_probability: out[node + sim * N], N * M floats
+   (REPLAY_E_UNSUPPORTED for a batch whose networks have different replica
+   counts). */
 int32_t replay_get_exposure(replay_ctx *ctx, int32_t k, float *out,
                             int64_t n_out);
 

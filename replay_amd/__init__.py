@@ -54,7 +54,7 @@ class _Network(C.Structure):
                 ("end", C.POINTER(C.c_int64)), ("initial_infected", C.c_float),
                 ("infect_prob", C.c_float), ("exposed_duration", C.c_int32),
                 ("infectious_duration", C.c_int32), ("resistant_duration", C.c_int32),
-                ("stream", C.c_uint32)]
+                ("stream", C.c_uint32), ("M", C.c_int32)]
 
 
 class _Info(C.Structure):
@@ -66,7 +66,7 @@ class _Info(C.Structure):
                 ("truncated_windows", C.c_int32), ("step_bytes", C.c_int64),
                 ("ring_depth", C.c_int32), ("n_networks", C.c_int32), ("variant", C.c_int32),
                 ("graph_ready", C.c_int32), ("entry_bytes", C.c_int32),
-                ("tune_ms", C.c_float * 4)]
+                ("replicas_total", C.c_int32), ("tune_ms", C.c_float * 4)]
 
 
 F_GRAPH = 1
@@ -84,7 +84,7 @@ EXPORTED_SYMBOLS = (
     "replay_open_batch", "replay_get_network_offsets", "replay_autotune", "replay_set_variant",
     "replay_marginal_create", "replay_marginal_forward", "replay_marginal_backward",
     "replay_marginal_destroy", "replay_generate_groups", "replay_contactgraph",
-    "replay_get_exposure_noalloc",
+    "replay_get_replica_offsets", "replay_get_network_state", "replay_get_exposure_noalloc",
 )
 
 _lib = None
@@ -125,6 +125,8 @@ def load_library() -> C.CDLL:
                                        P(ctx_p)]
     L.replay_open_batch.argtypes = [C.c_int32, P(_Network), P(_Params), P(ctx_p)]
     L.replay_get_network_offsets.argtypes = [ctx_p, P(C.c_int32), C.c_int32]
+    L.replay_get_replica_offsets.argtypes = [ctx_p, P(C.c_int32), C.c_int32]
+    L.replay_get_network_state.argtypes = [ctx_p, C.c_int32, P(C.c_int32), C.c_int64]
     L.replay_autotune.argtypes = [ctx_p, C.c_int32, P(C.c_int32)]
     L.replay_set_variant.argtypes = [ctx_p, C.c_int32]
     L.replay_get_info.argtypes = [ctx_p, P(_Info)]
@@ -166,6 +168,7 @@ def load_library() -> C.CDLL:
                  "replay_set_uniforms", "replay_get_slice", "replay_get_exposure",
                  "replay_get_exposure_noalloc",
                  "replay_open_batch", "replay_get_network_offsets", "replay_autotune",
+                 "replay_get_replica_offsets", "replay_get_network_state",
                  "replay_set_variant"):
         getattr(L, name).restype = C.c_int32
     if L.replay_abi_version() != ABI_VERSION:
@@ -219,6 +222,7 @@ class Network:
     infectious_duration: int = 0
     resistant_duration: int = 0
     stream: int = 0
+    M: int = 0  # replicas of this network; 0 takes the batch's Params.M
 
 
 def _ptr(a: np.ndarray, ctype):
@@ -238,6 +242,12 @@ class Simulation:
         off = np.zeros(self.n_networks + 1, np.int32)
         self._check(load_library().replay_get_network_offsets(ctx, _ptr(off, C.c_int32), off.size))
         self.node_offsets = off
+        roff = np.zeros(self.n_networks + 1, np.int32)
+        self._check(load_library().replay_get_replica_offsets(ctx, _ptr(roff, C.c_int32),
+                                                              roff.size))
+        self.replica_offsets = roff
+        # every network has M replicas (the counts and state layouts below)
+        self.uniform = bool((np.diff(roff) == self.M).all())
 
     # construction -------------------------------------------------------
     @classmethod
@@ -282,7 +292,8 @@ class Simulation:
             arr[k] = _Network(nw.n_nodes, None, len(a[0]), _ptr(a[0], C.c_int32),
                               _ptr(a[1], C.c_int32), _ptr(a[2], C.c_int64), _ptr(a[3], C.c_int64),
                               nw.initial_infected, nw.infect_prob, nw.exposed_duration,
-                              nw.infectious_duration, nw.resistant_duration, nw.stream)
+                              nw.infectious_duration, nw.resistant_duration, nw.stream,
+                              getattr(nw, "M", 0))
         ctx = C.c_void_p()
         p = params._c()
         rc = L.replay_open_batch(len(networks), arr, C.byref(p), C.byref(ctx))
@@ -343,14 +354,34 @@ class Simulation:
     def steps_done(self) -> int:
         return self.info()["steps_done"]
 
+    def _all_counts(self) -> np.ndarray:
+        k = self.steps_done
+        out = np.zeros((k + 1, int(self.replica_offsets[-1]), 4), np.int32)
+        self._check(load_library().replay_get_counts(self._ctx, _ptr(out, C.c_int32), out.size))
+        return out
+
     def counts(self) -> np.ndarray:
         """(steps_done+1, M, 4) int32 for one network, (steps_done+1,
         n_networks, M, 4) for a batch: exposed, infectious, resistant,
-        susceptible."""
-        k = self.steps_done
-        out = np.zeros((k + 1, self.n_networks, self.M, 4), np.int32)
-        self._check(load_library().replay_get_counts(self._ctx, _ptr(out, C.c_int32), out.size))
+        susceptible.  A batch whose networks have different replica counts
+        has no such array: use network_counts."""
+        if not self.uniform:
+            raise ValueError("networks have different replica counts: use network_counts(k)")
+        out = self._all_counts().reshape(-1, self.n_networks, self.M, 4)
         return out[:, 0] if self.n_networks == 1 else out
+
+    def network_counts(self, k: int) -> np.ndarray:
+        """(steps_done+1, M_k, 4) int32 counts of network k's replicas."""
+        r0, r1 = int(self.replica_offsets[k]), int(self.replica_offsets[k + 1])
+        return self._all_counts()[:, r0:r1].copy()
+
+    def network_state(self, k: int) -> np.ndarray:
+        """N_k*M_k int32 countdowns of network k, column-major (node + sim*N_k)."""
+        n = int(self.node_offsets[k + 1] - self.node_offsets[k])
+        out = np.zeros(n * int(self.replica_offsets[k + 1] - self.replica_offsets[k]), np.int32)
+        self._check(load_library().replay_get_network_state(self._ctx, k, _ptr(out, C.c_int32),
+                                                            out.size))
+        return out
 
     def autotune(self, steps: int = 16) -> int:
         v = C.c_int32(0)

@@ -30,7 +30,10 @@ struct Segs {  // runs of the tabulated class over n = 1..T for new infections
   int32_t cls[kMaxSegs];  // 1 E, 2 I, 3 R
 };
 
-// Constants of one network on the device.
+// Constants of one network on the device.  Its replicas occupy W words per
+// node: node i's first word is wsh + i * W (the network's words are one
+// contiguous run, so wsh is its first word less node_off * W), and replica r
+// is column moff + r of the per-replica count arrays.
 struct NetDev {
   float log_base;    // logf(1 - infect_prob), computed on the host
   float p0;          // initial_infected
@@ -38,6 +41,11 @@ struct NetDev {
   uint32_t stream;   // Philox counter word 3
   int32_t n_on, n_off, n_s;  // flip offsets of a new infection (kNever: none)
   int32_t m_on, m_off, m_s;  // the same for the initial seeds
+  int32_t M, W;      // replicas, and words per node
+  int32_t moff;      // first replica column
+  int32_t blk0;      // first block of update_kernel's grid
+  int64_t wsh;       // first word - node_off * W
+  int64_t woff, nwords;  // first word and word count of the network
 };
 
 struct StepArgs {
@@ -57,11 +65,18 @@ struct StepArgs {
   const int32_t *net_of;      // node -> network
   const uint32_t *rank_limit; // [n_net] of this window, or null (no cap bites)
   const float *p_tab;         // [n_net][tab_max+1]: P of c whole windows, or null
+  const int32_t *upd_net;     // network of each update_kernel block (-1: none)
   int32_t tab_max;
   int32_t long_row;           // rows with more entries go to long_row_kernel
   uint64_t seed;
-  int64_t NW;                 // N * W words
-  int32_t N, M, W, Mp, n_net, depth;
+  int64_t NW;                 // words of all networks
+  int32_t N;                  // nodes of all networks
+  int32_t M;                  // replicas per network when every network has the same
+                              // count (the injected-stream and exposure layouts), else 0
+  int32_t W;                  // most words per node of any network
+  int32_t Wg;                 // greatest common divisor of the networks' words per node
+  int32_t G;                  // replica columns of all networks (sum of their M)
+  int32_t n_net, depth;
   int32_t k;
   int32_t ws_rel, we_rel;
   float scale;
@@ -81,6 +96,9 @@ struct NetHost {
   int32_t node_off = 0;
   // timer tables
   std::vector<int32_t> lut_new, lut_init;  // c(n), n = 0..T
+  int32_t M = 0, W = 0;      // replicas and words per node
+  int32_t moff = 0;          // first replica column
+  int64_t woff = 0;          // first word
   int32_t n_on = rp::kNever, n_off = rp::kNever, n_s = rp::kNever;
   int32_t m_on = rp::kNever, m_off = rp::kNever, m_s = rp::kNever;
   rp::Segs segs{};
@@ -91,7 +109,13 @@ struct replay_ctx {
   replay_params p{};
   std::vector<NetHost> nets;
   int32_t n_net = 0;
-  int32_t N = 0, M = 0, W = 0, Mp = 0, T = 0, depth = 1;
+  // M: largest replica count of a network; W: its words per node; G: replica
+  // columns of all networks; NW: words of all networks; uniform: every
+  // network has M replicas (then node i's words start at i * W)
+  int32_t N = 0, M = 0, W = 0, Wg = 0, G = 0, T = 0, depth = 1;
+  int64_t NW = 0;
+  bool uniform = true;
+  int32_t upd_blocks = 0;  // blocks of update_kernel's grid that hold words
   // merged plan and slices
   std::vector<int64_t> iter_time;
   std::vector<int32_t> iter_window;
@@ -131,6 +155,8 @@ struct replay_ctx {
   rp::Segs *d_segs = nullptr;
   rp::NetDev *d_nets = nullptr;
   int32_t *d_net_of = nullptr;
+  int32_t *d_upd_net = nullptr;   // network of each update_kernel block
+  int32_t *d_col_net = nullptr;   // network of each replica column
   uint32_t *d_rank_limit = nullptr;  // [n_windows][n_net] when a cap bites
   float *d_p_tab = nullptr;          // [n_net][tab_max+1]
   int32_t *d_long_rows = nullptr;    // long rows of every epoch

@@ -44,6 +44,7 @@
 #include <atomic>
 #include <mutex>
 #include <climits>
+#include <numeric>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -279,10 +280,10 @@ __device__ __forceinline__ EntryFields fields(const v2u &en) {
 // with a nonzero word: a column's entries share one neighbour, so a column is
 // skipped whole or not at all, and adding nothing leaves the fp32 sum as it is.
 template <bool kIntMode, int WT, bool kFull, typename EnT>
-__device__ __forceinline__ void group_exposure(const StepArgs &a, int e0, int e1,
-                                               uint32_t rank_limit, int q0, int nq, int lane,
-                                               const uint64_t (&ss)[WT], const EnT &en0,
-                                               const EnT &en1, float (&E)[WT]) {
+__device__ __forceinline__ void group_exposure(const StepArgs &a, int64_t wsh, int W, int e0,
+                                               int e1, uint32_t rank_limit, int q0, int nq,
+                                               int lane, const uint64_t (&ss)[WT],
+                                               const EnT &en0, const EnT &en1, float (&E)[WT]) {
   uint32_t iacc[WT];
   float facc[WT];
 #pragma unroll
@@ -306,13 +307,15 @@ __device__ __forceinline__ void group_exposure(const StepArgs &a, int e0, int e1
     ov = o;
     col = o > 0 ? en.col : 0;
   };
-  // kFull: W is a multiple of WT, so every group is whole and, WT being
-  // even, the row offset col * W + q0 is even and 16-byte aligned.  Such a
+  // kFull: every network's W is a multiple of WT, so every group is whole
+  // and, WT being even, the word offset wsh + col * W + q0 is even (every
+  // network's words start at an even word) and 16-byte aligned.  Such a
   // group is read two words per load: a divergent gather costs the vector
   // L1 one tag lookup per lane and load, so 16-byte loads halve the lookups
-  // of 8-byte ones for the same bytes.
+  // of 8-byte ones for the same bytes.  The neighbour col belongs to the
+  // node's own network, so its words are at wsh + col * W.
   auto fetch = [&](int col, uint64_t (&raw)[WT]) {
-    const uint64_t *mrow = a.I_cur + static_cast<size_t>(col) * a.W + q0;
+    const uint64_t *mrow = a.I_cur + (wsh + static_cast<int64_t>(col) * W + q0);
     if (kFull && WT % 2 == 0) {
       const ulonglong2 *m2 = reinterpret_cast<const ulonglong2 *>(mrow);
 #pragma unroll
@@ -461,7 +464,7 @@ __device__ __forceinline__ bool try_integer_sums(const StepArgs &a, int e0, int 
 // which costs more than the arithmetic.  The Philox block serving words
 // 4g .. 4g+3 is computed once per lane and kept in pw / pw_grp.
 template <bool kInjected>
-__device__ __forceinline__ uint64_t draw_word(const StepArgs &a, const NetDev &nd, int net,
+__device__ __forceinline__ uint64_t draw_word(const StepArgs &a, const NetDev &nd, int64_t wi,
                                               int ii, int Q, int lane, uint64_t ss, float E,
                                               uint32_t (&pw)[4], int &pw_grp) {
   const int r = Q * kWave + lane;
@@ -508,8 +511,9 @@ __device__ __forceinline__ uint64_t draw_word(const StepArgs &a, const NetDev &n
       infect = 1.0f - rp::rp_expf(nd.log_base * (E / 3600.0f)) > u;
   }
   if (infect) {
-    a.entry[static_cast<size_t>(ii) * a.Mp + r] = 2 * a.k + 1;
-    atomicAdd(a.inf_count + (static_cast<size_t>(a.k) * a.n_net + net) * a.M + r, 1);
+    // entry is laid out [word][64 replicas]; wi is the node's first word
+    a.entry[(static_cast<size_t>(wi) + Q) * kWave + lane] = 2 * a.k + 1;
+    atomicAdd(a.inf_count + static_cast<size_t>(a.k) * a.G + nd.moff + r, 1);
   }
   return __ballot(infect);
 }
@@ -543,19 +547,23 @@ void expo_kernel(StepArgs a) {
   const int i = static_cast<int>(lb) * kWavesPerBlock +
                 __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
   if (i >= a.N) return;
-  uint64_t *X_k = a.X + static_cast<size_t>(a.k % a.depth) * a.NW + static_cast<size_t>(i) * a.W;
-  const uint64_t *S = a.S + static_cast<size_t>(i) * a.W;
   // The node's inputs are requested together before any is waited for: the
-  // row bounds and network (scalar), the first susceptible words and, once
-  // the bounds are in, the first two chunks of contact entries.
+  // row bounds and network (scalar), then the network's constants and the
+  // node's first susceptible words and, once the bounds are in, the first
+  // two chunks of contact entries.
   const int e0 = a.rowptr[i], e1 = a.rowptr[i + 1];
   const int net = a.net_of[i];
-  const uint64_t s_first = S[min(lane, min(WT, a.W) - 1)];
+  const NetDev nd = a.nets[net];
+  const int W = nd.W;
+  const int64_t wi = nd.wsh + static_cast<int64_t>(i) * W;  // the node's first word
+  uint64_t *X_k = a.X + static_cast<size_t>(a.k % a.depth) * a.NW + wi;
+  const uint64_t *S = a.S + wi;
+  const uint64_t s_first = S[min(lane, min(WT, W) - 1)];
   if (e1 - e0 > a.long_row) return;  // long_row_kernel takes it
   if (!kInjected && e1 <= e0) {
     // no contact in the epoch: no exposure, so no infection (a draw needs
     // E > 0); the node's words of ring slot k are zero
-    for (int w = lane; w < a.W; w += kWave) X_k[w] = 0ull;
+    for (int w = lane; w < W; w += kWave) X_k[w] = 0ull;
     return;
   }
   EnT en0{}, en1{};
@@ -568,10 +576,9 @@ void expo_kernel(StepArgs a) {
       if (WT <= 4 && e1 - e0 > kWave) en1 = load_chunk(a.entries, e0 + kWave, e1, lane);
     }
   }
-  const NetDev nd = a.nets[net];
   const uint32_t rlim = a.rank_limit ? a.rank_limit[net] : UINT32_MAX;
-  for (int q0 = 0; q0 < a.W; q0 += WT) {
-    const int nq = kFull ? WT : min(WT, a.W - q0);
+  for (int q0 = 0; q0 < W; q0 += WT) {
+    const int nq = kFull ? WT : min(WT, W - q0);
     // lane q holds susceptible word q0 + q (one load for the group)
     const uint64_t sv = q0 == 0 ? s_first : S[q0 + min(lane, nq - 1)];
     uint64_t ss[WT];
@@ -593,7 +600,8 @@ void expo_kernel(StepArgs a) {
         // rows of a few contacts keep the float path, which visits only the
         // active entries and costs less there than a column count per word
         if (e1 - e0 > kTryIntMinEntries && try_integer_sums(a, e0, e1)) {
-          group_exposure<true, WT, kFull, EnT>(a, e0, e1, rlim, q0, nq, lane, ss, en0, en1, E);
+          group_exposure<true, WT, kFull, EnT>(a, nd.wsh, W, e0, e1, rlim, q0, nq, lane, ss, en0,
+                                               en1, E);
           bool big = false;
 #pragma unroll
           for (int q = 0; q < WT; ++q)
@@ -602,13 +610,14 @@ void expo_kernel(StepArgs a) {
         }
       }
       if (!done)
-        group_exposure<kIntMode, WT, kFull, EnT>(a, e0, e1, rlim, q0, nq, lane, ss, en0, en1, E);
+        group_exposure<kIntMode, WT, kFull, EnT>(a, nd.wsh, W, e0, e1, rlim, q0, nq, lane, ss,
+                                                 en0, en1, E);
       uint32_t pw[4] = {0u, 0u, 0u, 0u};
       int pw_grp = -1;
 #pragma unroll
       for (int q = 0; q < WT; ++q)
         if (q < nq && ss[q] != 0ull)
-          x[q] = draw_word<kInjected>(a, nd, net, i, q0 + q, lane, ss[q], E[q], pw, pw_grp);
+          x[q] = draw_word<kInjected>(a, nd, wi, i, q0 + q, lane, ss[q], E[q], pw, pw_grp);
     }
     // lane q stores word q of the group
     uint64_t mine = 0ull;
@@ -634,6 +643,8 @@ void expo_kernel(StepArgs a) {
 template <bool kIntMode, bool kInjected, bool kTryInt>
 __global__ __launch_bounds__(kBlock) void long_row_kernel(StepArgs a, const int32_t *rows,
                                                           int n_rows) {
+  // a.W (the most words per node of any network) wavefronts per listed row;
+  // those past the row's own network's W have nothing to do
   const int lane = threadIdx.x & (kWave - 1);
   const int wid = static_cast<int>(blockIdx.x) * kWavesPerBlock +
                   __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
@@ -641,30 +652,34 @@ __global__ __launch_bounds__(kBlock) void long_row_kernel(StepArgs a, const int3
   const int h = wid / a.W;
   const int q = wid - h * a.W;
   const int i = rows[h];
-  uint64_t *X_k = a.X + static_cast<size_t>(a.k % a.depth) * a.NW + static_cast<size_t>(i) * a.W;
-  const uint64_t ss[1] = {a.S[static_cast<size_t>(i) * a.W + q]};
+  const int net = a.net_of[i];
+  const NetDev nd = a.nets[net];
+  if (q >= nd.W) return;
+  const int64_t wi = nd.wsh + static_cast<int64_t>(i) * nd.W;
+  uint64_t *X_k = a.X + static_cast<size_t>(a.k % a.depth) * a.NW + wi;
+  const uint64_t ss[1] = {a.S[wi + q]};
   uint64_t x = 0ull;
   if (ss[0] != 0ull) {
     const int e0 = a.rowptr[i], e1 = a.rowptr[i + 1];
-    const int net = a.net_of[i];
-    const NetDev nd = a.nets[net];
     const uint32_t rlim = a.rank_limit ? a.rank_limit[net] : UINT32_MAX;
     float E[1];
     const v4i en0 = load_chunk(a.entries, e0, e1, lane);  // rows here hold > 2 chunks
     const v4i en1 = load_chunk(a.entries, e0 + kWave, e1, lane);
     bool done = false;
     if (kTryInt && try_integer_sums(a, e0, e1)) {
-      group_exposure<true, 1, false, v4i>(a, e0, e1, rlim, q, 1, lane, ss, en0, en1, E);
+      group_exposure<true, 1, false, v4i>(a, nd.wsh, nd.W, e0, e1, rlim, q, 1, lane, ss, en0,
+                                          en1, E);
       // E is the integer sum converted to float: >= 2^24 exactly when the
       // sum is
       const bool sus = __builtin_amdgcn_inverse_ballot_w64(ss[0]);
       done = __ballot(sus && E[0] >= 16777216.0f) == 0ull;
     }
     if (!done)
-      group_exposure<kIntMode, 1, false, v4i>(a, e0, e1, rlim, q, 1, lane, ss, en0, en1, E);
+      group_exposure<kIntMode, 1, false, v4i>(a, nd.wsh, nd.W, e0, e1, rlim, q, 1, lane, ss, en0,
+                                              en1, E);
     uint32_t pw[4] = {0u, 0u, 0u, 0u};
     int pw_grp = -1;
-    x = draw_word<kInjected>(a, nd, net, i, q, lane, ss[0], E[0], pw, pw_grp);
+    x = draw_word<kInjected>(a, nd, wi, i, q, lane, ss[0], E[0], pw, pw_grp);
   }
   if (lane == 0) X_k[q] = x;
 }
@@ -681,7 +696,9 @@ __device__ __forceinline__ void csa(uint64_t &hi, uint64_t &lo, uint64_t a, uint
 constexpr int kPartialWords = 256;  // partial-overlap words recorded per wavefront
 constexpr int kTabLds = 1024;       // probability-table entries staged in LDS per block
 
-// Phase 1, bit-sliced form (integer windows only).  A wavefront covers
+// Phase 1, bit-sliced form (integer windows only; contexts whose networks
+// all have the same replica count, so node i's words start at i * W).  A
+// wavefront covers
 // G = 64 / W nodes with one lane per (node, 64-replica word).  Each lane walks
 // its node's contact entries eight at a time, gathers the neighbour's word for
 // every whole-window contact and adds the eight words into bit-sliced
@@ -908,8 +925,10 @@ __global__ __launch_bounds__(kBlock) void sliced_kernel(StepArgs a, int G) {
           infect = P > u;
         }
         if (infect) {
-          a.entry[static_cast<size_t>(i) * a.Mp + r] = 2 * a.k + 1;
-          atomicAdd(a.inf_count + (static_cast<size_t>(a.k) * a.n_net + net) * a.M + r, 1);
+          // every network has a.M replicas here: node i's words start at i * W
+          a.entry[static_cast<size_t>(i) * W * kWave + r] = 2 * a.k + 1;
+          atomicAdd(a.inf_count + static_cast<size_t>(a.k) * a.G + static_cast<size_t>(net) * a.M + r,
+                    1);
         }
         const uint64_t x = __ballot(infect);
         if (lane == q) mine = x;
@@ -928,12 +947,16 @@ constexpr size_t sliced_lds_bytes(int nb) {
 
 // Phase 2 of an iteration: one thread per word applies the timer flips for
 // iteration k+1 and advances the running prefix of the infection counts,
-// P[k] = P[k-1] + inf_count[k-1].
+// P[k] = P[k-1] + inf_count[k-1].  Each block's words belong to one network
+// (a.upd_net names it; a network's words start at its own block), so the
+// flip offsets are the block's.
 __global__ __launch_bounds__(kBlock) void update_kernel(StepArgs a) {
-  const int64_t w = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
-  if (w < a.NW) {
-    const int i = static_cast<int>(w / a.W);
-    const NetDev nd = a.nets[a.net_of[i]];
+  const int net = a.upd_net[blockIdx.x];
+  if (net >= 0) {
+    const NetDev nd = a.nets[net];
+    const int64_t j = static_cast<int64_t>(blockIdx.x - nd.blk0) * kBlock + threadIdx.x;
+    const int64_t w = nd.woff + j;
+    if (j < nd.nwords) {
     const uint64_t x = a.X[static_cast<size_t>(a.k % a.depth) * a.NW + w];
     uint64_t inf = a.I_cur[w];
     uint64_t sus = a.S[w] & ~x;
@@ -955,11 +978,13 @@ __global__ __launch_bounds__(kBlock) void update_kernel(StepArgs a) {
     }
     a.I_next[w] = inf;
     a.S[w] = sus;
+    }
   }
-  const int64_t G = static_cast<int64_t>(a.n_net) * a.M;
-  if (a.k >= 1 && w < G)
-    a.P[static_cast<size_t>(a.k) * G + w] =
-        a.P[static_cast<size_t>(a.k - 1) * G + w] + a.inf_count[static_cast<size_t>(a.k - 1) * G + w];
+  const int64_t g = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
+  const int64_t G = a.G;
+  if (a.k >= 1 && g < G)
+    a.P[static_cast<size_t>(a.k) * G + g] =
+        a.P[static_cast<size_t>(a.k - 1) * G + g] + a.inf_count[static_cast<size_t>(a.k - 1) * G + g];
 }
 
 struct InitArgs {
@@ -967,26 +992,29 @@ struct InitArgs {
   uint64_t *I;
   uint64_t *X_init;
   int32_t *entry;
-  int32_t *init_count;    // [n_net][M]
-  const float *uniforms;  // slot 0 or null
+  int32_t *init_count;    // [G] replica columns
+  const float *uniforms;  // slot 0 or null ([sim][node], every network with M replicas)
   const NetDev *nets;
   const int32_t *net_of;
   uint64_t seed;
-  int32_t N, M, W, Mp;
+  int32_t N;
 };
 
 // cu:73-90 with a counter-based draw (slot 0): seeds, initial bit words,
-// entries and per-replica seed counts.
+// entries and per-replica seed counts.  blockIdx.y is the word; words past
+// the node's own network's W have nothing to do.
 __global__ __launch_bounds__(kBlock) void init_kernel(InitArgs a) {
   const int lane = threadIdx.x & (kWave - 1);
   const int wave = threadIdx.x / kWave;
   const int q = blockIdx.y;
   const int r = q * kWave + lane;
-  const bool rv = r < a.M;
   const int i = blockIdx.x * kWavesPerBlock + wave;
   if (i >= a.N) return;
   const int net = a.net_of[i];
   const NetDev nd = a.nets[net];
+  if (q >= nd.W) return;
+  const bool rv = r < nd.M;
+  const size_t w = static_cast<size_t>(nd.wsh + static_cast<int64_t>(i) * nd.W + q);
   bool seed = false;
   if (rv) {
     const float u = a.uniforms ? a.uniforms[static_cast<size_t>(r) * a.N + i]
@@ -996,10 +1024,9 @@ __global__ __launch_bounds__(kBlock) void init_kernel(InitArgs a) {
   }
   const uint64_t xi = __ballot(seed);
   const uint64_t valid = __ballot(rv);
-  a.entry[static_cast<size_t>(i) * a.Mp + r] = seed ? 0 : -1;
-  if (seed) atomicAdd(a.init_count + static_cast<size_t>(net) * a.M + r, 1);
+  a.entry[w * kWave + lane] = seed ? 0 : -1;
+  if (seed) atomicAdd(a.init_count + nd.moff + r, 1);
   if (lane == 0) {
-    const size_t w = static_cast<size_t>(i) * a.W + q;
     a.X_init[w] = xi;
     a.S[w] = nd.m_s == 0 ? valid : (valid & ~xi);
     a.I[w] = nd.m_on == 0 ? xi : 0ull;
@@ -1044,15 +1071,15 @@ __global__ void prefix_tail_kernel(int32_t *P, const int32_t *inf_count, int32_t
 }
 
 // Counts of blocks j0..j1: exposed, infectious, resistant of the state after
-// iteration j-1 (block 0: the initial state), per network and replica.
+// iteration j-1 (block 0: the initial state), per replica column g (network
+// col_net[g], replica g - its first column).
 __global__ void counts_kernel(int32_t *counts, const int32_t *P, const int32_t *init_count,
-                              const int8_t *class_init, const Segs *segs, int32_t n_net,
-                              int32_t M, int32_t T, int32_t j0, int32_t j1) {
-  const int g = blockIdx.x * blockDim.x + threadIdx.x;  // net * M + r
+                              const int8_t *class_init, const Segs *segs, const int32_t *col_net,
+                              int32_t G, int32_t T, int32_t j0, int32_t j1) {
+  const int g = blockIdx.x * blockDim.x + threadIdx.x;
   const int j = j0 + blockIdx.y;
-  const int G = n_net * M;
   if (g >= G || j > j1) return;
-  const int net = g / M;
+  const int net = col_net[g];
   const Segs &sg = segs[net];
   int32_t c[4] = {0, 0, 0, 0};
   for (int h = 0; h < sg.n; ++h) {
@@ -1150,7 +1177,7 @@ void release(replay_ctx *ctx) {
                   ctx->d_X,       ctx->d_Xinit,  ctx->d_entry,  ctx->d_inf,         ctx->d_P,
                   ctx->d_init_count, ctx->d_counts, ctx->d_class_init, ctx->d_segs, ctx->d_nets,
                   ctx->d_net_of,  ctx->d_rank_limit, ctx->d_uniforms, ctx->d_p_tab,
-                  ctx->d_long_rows};
+                  ctx->d_long_rows, ctx->d_upd_net, ctx->d_col_net};
   for (void *b : bufs) (void)hipFree(b);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
   ctx->stream = nullptr;
@@ -1242,14 +1269,16 @@ StepArgs rp::step_args(replay_ctx *ctx, int32_t k) {
   a.net_of = ctx->d_net_of;
   a.rank_limit = ctx->any_cut ? ctx->d_rank_limit + static_cast<size_t>(win) * ctx->n_net : nullptr;
   a.p_tab = ctx->d_p_tab;
+  a.upd_net = ctx->d_upd_net;
   a.tab_max = ctx->tab_max;
   a.long_row = rp::kLongRow;
   a.seed = ctx->p.seed;
-  a.NW = static_cast<int64_t>(ctx->N) * ctx->W;
+  a.NW = ctx->NW;
   a.N = ctx->N;
-  a.M = ctx->M;
+  a.M = ctx->uniform ? ctx->M : 0;
   a.W = ctx->W;
-  a.Mp = ctx->Mp;
+  a.Wg = ctx->Wg;
+  a.G = ctx->G;
   a.n_net = ctx->n_net;
   a.depth = ctx->depth;
   a.k = k;
@@ -1271,17 +1300,18 @@ void launch_expo_t(const StepArgs &a, int variant, hipStream_t st) {
   // and, where the context holds the 8-byte copy of the entries, the one
   // that streams it
   const bool c8 = a.entries8 != nullptr;
+  // (every network's W a multiple of the group width: a.Wg, their gcd)
   if (variant == REPLAY_V_NODE16) {
-    if (a.W % 16 == 0 && c8)
+    if (a.Wg % 16 == 0 && c8)
       hipLaunchKernelGGL((expo_kernel<kInt, kInj, 16, true, true, kTry>), grid, dim3(kBlock), 0, st, a);
-    else if (a.W % 16 == 0)
+    else if (a.Wg % 16 == 0)
       hipLaunchKernelGGL((expo_kernel<kInt, kInj, 16, true, false, kTry>), grid, dim3(kBlock), 0, st, a);
     else
       hipLaunchKernelGGL((expo_kernel<kInt, kInj, 16, false, false, kTry>), grid, dim3(kBlock), 0, st, a);
   } else if (variant == REPLAY_V_NODE4) {
-    if (a.W % 4 == 0 && c8)
+    if (a.Wg % 4 == 0 && c8)
       hipLaunchKernelGGL((expo_kernel<kInt, kInj, 4, true, true, kTry>), grid, dim3(kBlock), 0, st, a);
-    else if (a.W % 4 == 0)
+    else if (a.Wg % 4 == 0)
       hipLaunchKernelGGL((expo_kernel<kInt, kInj, 4, true, false, kTry>), grid, dim3(kBlock), 0, st, a);
     else
       hipLaunchKernelGGL((expo_kernel<kInt, kInj, 4, false, false, kTry>), grid, dim3(kBlock), 0, st, a);
@@ -1386,15 +1416,15 @@ void launch_step(const StepArgs &a, int variant, bool int_mode, bool injected, d
 }
 
 dim3 step_grid(const replay_ctx *ctx) {
-  // threads cover every word and, for the running prefix, every replica
-  const int64_t NW = static_cast<int64_t>(ctx->N) * ctx->W;
-  const int64_t cover = std::max<int64_t>(NW, static_cast<int64_t>(ctx->n_net) * ctx->M);
-  return dim3(static_cast<unsigned>((cover + kBlock - 1) / kBlock));
+  // blocks cover every network's words (upd_blocks) and, for the running
+  // prefix, every replica column
+  const int64_t cover = std::max<int64_t>(ctx->upd_blocks, (ctx->G + kBlock - 1) / kBlock);
+  return dim3(static_cast<unsigned>(cover));
 }
 
 bool variant_applies(const replay_ctx *ctx, int v) {
   if (v == REPLAY_V_SLICED)
-    return ctx->W <= kWave && ctx->N < (1 << 24) &&
+    return ctx->uniform && ctx->W <= kWave && ctx->N < (1 << 24) &&
            static_cast<int64_t>(ctx->N) * ctx->W < (int64_t(1) << 32);
   return v == REPLAY_V_WORD || v == REPLAY_V_NODE4 || v == REPLAY_V_NODE16;
 }
@@ -1419,7 +1449,7 @@ int enqueue_steps(replay_ctx *ctx, int32_t steps) {
 }
 
 int do_reset(replay_ctx *ctx) {
-  const size_t G = static_cast<size_t>(ctx->n_net) * ctx->M;
+  const size_t G = static_cast<size_t>(ctx->G);
   const size_t T = static_cast<size_t>(ctx->T);
   HIP_OR_FAIL(ctx, hipMemsetAsync(ctx->d_inf, 0, (T + 1) * G * sizeof(int32_t), ctx->stream));
   HIP_OR_FAIL(ctx, hipMemsetAsync(ctx->d_P, 0, (T + 2) * G * sizeof(int32_t), ctx->stream));
@@ -1436,9 +1466,6 @@ int do_reset(replay_ctx *ctx) {
   a.net_of = ctx->d_net_of;
   a.seed = ctx->p.seed;
   a.N = ctx->N;
-  a.M = ctx->M;
-  a.W = ctx->W;
-  a.Mp = ctx->Mp;
   dim3 grid((ctx->N + kWavesPerBlock - 1) / kWavesPerBlock, ctx->W);
   hipLaunchKernelGGL(init_kernel, grid, dim3(kBlock), 0, ctx->stream, a);
   HIP_OR_FAIL(ctx, hipGetLastError());
@@ -1452,13 +1479,13 @@ int do_reset(replay_ctx *ctx) {
 int enqueue_counts(replay_ctx *ctx) {
   const int32_t j0 = ctx->counts_done + 1, j1 = ctx->steps_done;
   if (j0 > j1) return REPLAY_OK;
-  const int G = ctx->n_net * ctx->M;
+  const int G = ctx->G;
   const int tb = 256, nb = (G + tb - 1) / tb;
   hipLaunchKernelGGL(prefix_tail_kernel, dim3(nb), dim3(tb), 0, ctx->stream, ctx->d_P, ctx->d_inf,
                      G, j1);
   hipLaunchKernelGGL(counts_kernel, dim3(nb, j1 - j0 + 1), dim3(tb), 0, ctx->stream,
                      ctx->d_counts, ctx->d_P, ctx->d_init_count, ctx->d_class_init, ctx->d_segs,
-                     ctx->n_net, ctx->M, ctx->T, j0, j1);
+                     ctx->d_col_net, G, ctx->T, j0, j1);
   HIP_OR_FAIL(ctx, hipGetLastError());
   ctx->counts_done = j1;
   return REPLAY_OK;
@@ -1526,9 +1553,6 @@ int finish_open(replay_ctx *ctx) {
   std::string err;
   const replay_params &p = ctx->p;
   ctx->T = p.iterations;
-  ctx->M = p.M;
-  ctx->W = (p.M + kWave - 1) / kWave;
-  ctx->Mp = ctx->W * kWave;
   ctx->n_net = static_cast<int32_t>(ctx->nets.size());
   // per network, independent, on host threads: rank order, cap, windows,
   // timers
@@ -1551,18 +1575,47 @@ int finish_open(replay_ctx *ctx) {
   for (size_t n = 0; n < n_nets; ++n)
     if (rcs[n] != REPLAY_OK)
       return fail(ctx, rcs[n], (n_nets > 1 ? "network " + std::to_string(n) + ": " : "") + errs[n]);
+  // node rows, words and replica columns: network after network
   int32_t off = 0;
+  int64_t woff = 0, moff = 0;
   for (auto &nh : ctx->nets) {
     nh.node_off = off;
     off += nh.net.n_nodes;
+    nh.W = (nh.M + kWave - 1) / kWave;
+    nh.woff = woff;
+    woff += static_cast<int64_t>(nh.net.n_nodes) * nh.W;
+    nh.moff = static_cast<int32_t>(std::min<int64_t>(moff, INT32_MAX));
+    moff += nh.M;
+    ctx->M = std::max(ctx->M, nh.M);
+    ctx->W = std::max(ctx->W, nh.W);
+    ctx->Wg = std::gcd(ctx->Wg, nh.W);
+    ctx->uniform = ctx->uniform && nh.M == ctx->nets[0].M;
     for (int32_t n : {nh.n_on, nh.n_off, nh.n_s})
       if (n != kNever) ctx->depth = std::max(ctx->depth, n);
     ctx->n_contacts += static_cast<int64_t>(nh.net.contacts.size());
     ctx->nlists = std::max(ctx->nlists, nh.net.nlists);
   }
   ctx->N = off;
-  if (static_cast<int64_t>(ctx->N) * ctx->W > INT32_MAX)
-    return fail(ctx, REPLAY_E_UNSUPPORTED, "more than 2^31 replica words");
+  ctx->NW = woff;
+  if (woff > INT32_MAX) return fail(ctx, REPLAY_E_UNSUPPORTED, "more than 2^31 replica words");
+  if (moff > INT32_MAX / 4) return fail(ctx, REPLAY_E_UNSUPPORTED, "more than 2^29 replicas");
+  ctx->G = static_cast<int32_t>(moff);
+  // update_kernel's blocks: each network's words start at a block of their own
+  std::vector<int32_t> blk0(static_cast<size_t>(ctx->n_net));
+  std::vector<int32_t> upd_net;
+  for (int32_t n = 0; n < ctx->n_net; ++n) {
+    const NetHost &nh = ctx->nets[n];
+    blk0[n] = static_cast<int32_t>(upd_net.size());
+    const int64_t nb = (static_cast<int64_t>(nh.net.n_nodes) * nh.W + kBlock - 1) / kBlock;
+    upd_net.insert(upd_net.end(), static_cast<size_t>(nb), n);
+  }
+  ctx->upd_blocks = static_cast<int32_t>(upd_net.size());
+  upd_net.resize(std::max<size_t>(upd_net.size(), static_cast<size_t>((ctx->G + kBlock - 1) / kBlock)),
+                 -1);
+  std::vector<int32_t> col_net(static_cast<size_t>(ctx->G));
+  for (int32_t n = 0; n < ctx->n_net; ++n)
+    std::fill(col_net.begin() + ctx->nets[n].moff, col_net.begin() + ctx->nets[n].moff + ctx->nets[n].M,
+              n);
   // windows are shared: every network planned the same iterations
   const rp::Plan &p0 = ctx->nets[0].plan;
   ctx->iter_time = p0.iter_time;
@@ -1687,8 +1740,11 @@ int finish_open(replay_ctx *ctx) {
   std::vector<int32_t> net_of(static_cast<size_t>(ctx->N));
   for (int32_t n = 0; n < ctx->n_net; ++n) {
     const NetHost &nh = ctx->nets[n];
-    nd[n] = NetDev{nh.log_base, nh.p0, nh.node_off, nh.stream, nh.n_on, nh.n_off, nh.n_s,
-                   nh.m_on, nh.m_off, nh.m_s};
+    nd[n] = NetDev{nh.log_base, nh.p0,      nh.node_off, nh.stream, nh.n_on,
+                   nh.n_off,    nh.n_s,     nh.m_on,     nh.m_off,  nh.m_s,
+                   nh.M,        nh.W,       nh.moff,     blk0[n],
+                   nh.woff - static_cast<int64_t>(nh.node_off) * nh.W,
+                   nh.woff,     static_cast<int64_t>(nh.net.n_nodes) * nh.W};
     segs[n] = nh.segs;
     std::copy(nh.class_init.begin(), nh.class_init.end(),
               class_init.begin() + static_cast<size_t>(n) * (ctx->T + 1));
@@ -1696,9 +1752,9 @@ int finish_open(replay_ctx *ctx) {
   }
   HIP_OR_FAIL(ctx, hipSetDevice(p.device));
   HIP_OR_FAIL(ctx, hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
-  const size_t NW = static_cast<size_t>(ctx->N) * ctx->W;
+  const size_t NW = static_cast<size_t>(ctx->NW);
   const size_t T = static_cast<size_t>(ctx->T);
-  const size_t G = static_cast<size_t>(ctx->n_net) * ctx->M;
+  const size_t G = static_cast<size_t>(ctx->G);
   int rc;
   if ((rc = upload(ctx, &ctx->d_entries, reinterpret_cast<const int4 *>(entries.data()),
                    entries.size())) ||
@@ -1707,6 +1763,8 @@ int finish_open(replay_ctx *ctx) {
       (rc = upload(ctx, &ctx->d_segs, segs.data(), segs.size())) ||
       (rc = upload(ctx, &ctx->d_class_init, class_init.data(), class_init.size())) ||
       (rc = upload(ctx, &ctx->d_net_of, net_of.data(), net_of.size())) ||
+      (rc = upload(ctx, &ctx->d_upd_net, upd_net.data(), upd_net.size())) ||
+      (rc = upload(ctx, &ctx->d_col_net, col_net.data(), col_net.size())) ||
       (rc = (ctx->any_cut ? upload(ctx, &ctx->d_rank_limit, limits.data(), limits.size())
                           : REPLAY_OK)) ||
       (rc = upload(ctx, &ctx->d_long_rows, long_rows.data(), long_rows.size())) ||
@@ -1714,7 +1772,7 @@ int finish_open(replay_ctx *ctx) {
       (rc = dev_alloc(ctx, &ctx->d_I[1], NW)) ||
       (rc = dev_alloc(ctx, &ctx->d_X, NW * static_cast<size_t>(ctx->depth))) ||
       (rc = dev_alloc(ctx, &ctx->d_Xinit, NW)) ||
-      (rc = dev_alloc(ctx, &ctx->d_entry, static_cast<size_t>(ctx->N) * ctx->Mp)) ||
+      (rc = dev_alloc(ctx, &ctx->d_entry, NW * kWave)) ||
       (rc = dev_alloc(ctx, &ctx->d_inf, (T + 1) * G)) ||
       (rc = dev_alloc(ctx, &ctx->d_P, (T + 2) * G)) ||
       (rc = dev_alloc(ctx, &ctx->d_init_count, G)) ||
@@ -1788,6 +1846,7 @@ int open_common(replay_ctx *ctx, replay_ctx **out) {
 
 NetHost single_net(const replay_params *p) {
   NetHost nh;
+  nh.M = p->M;
   nh.p0 = p->initial_infected;
   nh.beta = p->infect_prob;
   nh.Ed = p->exposed_duration;
@@ -1817,6 +1876,15 @@ int noalloc_scratch(replay_ctx *ctx, float **ptr) {
   }
   *ptr = addr[dev];
   return REPLAY_OK;
+}
+
+// Countdown of a replica after k iterations from its entry value (-1 never
+// infected, 0 initial seed, 2j+1 infected at iteration j), by the network's
+// tabulated sequences.
+int32_t countdown_of(const NetHost &nh, int32_t s, int32_t k) {
+  if (s == 0) return nh.lut_init[k];
+  if (s > 0) return nh.lut_new[k - (s - 1) / 2];
+  return 0;
 }
 
 // Captures the step and counts kernels of a full run (iterations 0..T-1
@@ -1918,6 +1986,8 @@ int32_t replay_open_batch(int32_t n_networks, const replay_network *nets, const 
     rc = check_durations(nets[n].exposed_duration, nets[n].infectious_duration,
                          nets[n].resistant_duration);
     if (rc != REPLAY_OK) return rc;
+    if (nets[n].M < 0)
+      return fail(nullptr, REPLAY_E_ARG, "network " + std::to_string(n) + ": M must be >= 0");
   }
   replay_ctx *ctx = new (std::nothrow) replay_ctx();
   if (!ctx) return fail(nullptr, REPLAY_E_NOMEM, "out of host memory");
@@ -1926,6 +1996,7 @@ int32_t replay_open_batch(int32_t n_networks, const replay_network *nets, const 
   for (int32_t n = 0; n < n_networks; ++n) {
     const replay_network &d = nets[n];
     NetHost &nh = ctx->nets[n];
+    nh.M = d.M > 0 ? d.M : p->M;
     nh.p0 = d.initial_infected;
     nh.beta = d.infect_prob;
     nh.Ed = d.exposed_duration;
@@ -1955,6 +2026,7 @@ int32_t replay_get_info(replay_ctx *ctx, replay_info *out) {
   std::memset(out, 0, sizeof *out);
   out->n_nodes = ctx->N;
   out->M = ctx->M;
+  out->replicas_total = ctx->G;
   out->n_contacts = ctx->n_contacts;
   out->nlists = ctx->nlists;
   out->iterations = ctx->T;
@@ -1964,8 +2036,8 @@ int32_t replay_get_info(replay_ctx *ctx, replay_info *out) {
   out->n_entries = ctx->n_entries;
   out->slice_bytes = ctx->n_entries * static_cast<int64_t>(sizeof(rp::Entry)) +
                      static_cast<int64_t>(ctx->epoch_t0.size()) * (ctx->N + 1) * 4;
-  const int64_t NW = static_cast<int64_t>(ctx->N) * ctx->W;
-  out->state_bytes = NW * 8 * (4 + ctx->depth) + static_cast<int64_t>(ctx->N) * ctx->Mp * 4;
+  const int64_t NW = ctx->NW;
+  out->state_bytes = NW * 8 * (4 + ctx->depth) + NW * kWave * 4;
   out->steps_done = ctx->steps_done;
   out->truncated_windows = ctx->truncated;
   // per word: S read + write, own I read, I_next write, X_k write, and one
@@ -1979,11 +2051,11 @@ int32_t replay_get_info(replay_ctx *ctx, replay_info *out) {
                                            : static_cast<double>(ctx->n_entries) /
                                                  static_cast<double>(ctx->epoch_t0.size());
   // entries at the width the NODE4 / NODE16 kernels of whole groups stream
-  const bool c8 = ctx->d_entries8 && ((ctx->variant == REPLAY_V_NODE4 && ctx->W % 4 == 0) ||
-                                      (ctx->variant == REPLAY_V_NODE16 && ctx->W % 16 == 0));
+  const bool c8 = ctx->d_entries8 && ((ctx->variant == REPLAY_V_NODE4 && ctx->Wg % 4 == 0) ||
+                                      (ctx->variant == REPLAY_V_NODE16 && ctx->Wg % 16 == 0));
   out->step_bytes = NW * 8 * (5 + ring_reads) + static_cast<int64_t>((c8 ? 8.0 : 16.0) * ent) +
                     4 * (static_cast<int64_t>(ctx->N) + 1) +
-                    12 * static_cast<int64_t>(ctx->n_net) * ctx->M;
+                    12 * static_cast<int64_t>(ctx->G);
   out->ring_depth = ctx->depth;
   out->n_networks = ctx->n_net;
   out->variant = ctx->variant;
@@ -2105,7 +2177,7 @@ int32_t replay_sync(replay_ctx *ctx) {
 
 int32_t replay_get_counts(replay_ctx *ctx, int32_t *out, int64_t n_out) {
   if (!ctx || !out) return REPLAY_E_ARG;
-  const int64_t G = static_cast<int64_t>(ctx->n_net) * ctx->M;
+  const int64_t G = ctx->G;
   const int64_t n = static_cast<int64_t>(ctx->steps_done + 1) * G * 4;
   if (n_out < n) return fail(ctx, REPLAY_E_ARG, "counts buffer too small");
   HIP_OR_FAIL(ctx, hipSetDevice(ctx->p.device));
@@ -2114,36 +2186,64 @@ int32_t replay_get_counts(replay_ctx *ctx, int32_t *out, int64_t n_out) {
   HIP_OR_FAIL(ctx, hipMemcpyAsync(out, ctx->d_counts, static_cast<size_t>(n) * sizeof(int32_t),
                                   hipMemcpyDeviceToHost, ctx->stream));
   HIP_OR_FAIL(ctx, hipStreamSynchronize(ctx->stream));
-  for (int64_t b = 0; b < n; b += 4) {
-    const int32_t net = static_cast<int32_t>((b / 4 / ctx->M) % ctx->n_net);
-    out[b + 3] = ctx->nets[net].net.n_nodes - out[b] - out[b + 1] - out[b + 2];
-  }
+  std::vector<int32_t> nodes(static_cast<size_t>(G));  // nodes of each column's network
+  for (const NetHost &nh : ctx->nets)
+    std::fill(nodes.begin() + nh.moff, nodes.begin() + nh.moff + nh.M, nh.net.n_nodes);
+  for (int64_t b = 0; b < n; b += 4)
+    out[b + 3] = nodes[static_cast<size_t>((b / 4) % G)] - out[b] - out[b + 1] - out[b + 2];
   return REPLAY_OK;
 }
 
 int32_t replay_get_state(replay_ctx *ctx, int32_t *out, int64_t n_out) {
   if (!ctx || !out) return REPLAY_E_ARG;
+  if (!ctx->uniform)
+    return fail(ctx, REPLAY_E_UNSUPPORTED,
+                "networks have different replica counts: use replay_get_network_state");
   const int64_t n = static_cast<int64_t>(ctx->N) * ctx->M;
   if (n_out < n) return fail(ctx, REPLAY_E_ARG, "state buffer too small");
   HIP_OR_FAIL(ctx, hipSetDevice(ctx->p.device));
-  std::vector<int32_t> ent(static_cast<size_t>(ctx->N) * ctx->Mp);
+  // entry is [word][64 replicas]; with one replica count node i's words
+  // start at i * W, so this is [node][W * 64]
+  std::vector<int32_t> ent(static_cast<size_t>(ctx->NW) * kWave);
   HIP_OR_FAIL(ctx, hipMemcpyAsync(ent.data(), ctx->d_entry, ent.size() * sizeof(int32_t),
                                   hipMemcpyDeviceToHost, ctx->stream));
   HIP_OR_FAIL(ctx, hipStreamSynchronize(ctx->stream));
   const int32_t k = ctx->steps_done;
+  const size_t Mp = static_cast<size_t>(ctx->W) * kWave;
   for (const NetHost &nh : ctx->nets) {
-    const auto &ln = nh.lut_new, &li = nh.lut_init;
     for (int32_t i = nh.node_off; i < nh.node_off + nh.net.n_nodes; ++i)
-      for (int32_t r = 0; r < ctx->M; ++r) {
-        const int32_t s = ent[static_cast<size_t>(i) * ctx->Mp + r];
-        int32_t c = 0;
-        if (s == 0)
-          c = li[k];
-        else if (s > 0)
-          c = ln[k - (s - 1) / 2];
-        out[static_cast<size_t>(r) * ctx->N + i] = c;
-      }
+      for (int32_t r = 0; r < ctx->M; ++r)
+        out[static_cast<size_t>(r) * ctx->N + i] =
+            countdown_of(nh, ent[static_cast<size_t>(i) * Mp + r], k);
   }
+  return REPLAY_OK;
+}
+
+int32_t replay_get_network_state(replay_ctx *ctx, int32_t net, int32_t *out, int64_t n_out) {
+  if (!ctx || !out || net < 0 || net >= ctx->n_net) return REPLAY_E_ARG;
+  const NetHost &nh = ctx->nets[net];
+  const int32_t Nn = nh.net.n_nodes;
+  const int64_t n = static_cast<int64_t>(Nn) * nh.M;
+  if (n_out < n) return fail(ctx, REPLAY_E_ARG, "state buffer too small");
+  HIP_OR_FAIL(ctx, hipSetDevice(ctx->p.device));
+  // the network's words are one run of Nn * W words from woff
+  std::vector<int32_t> ent(static_cast<size_t>(Nn) * nh.W * kWave);
+  HIP_OR_FAIL(ctx, hipMemcpyAsync(ent.data(), ctx->d_entry + static_cast<size_t>(nh.woff) * kWave,
+                                  ent.size() * sizeof(int32_t), hipMemcpyDeviceToHost,
+                                  ctx->stream));
+  HIP_OR_FAIL(ctx, hipStreamSynchronize(ctx->stream));
+  const int32_t k = ctx->steps_done;
+  const size_t Mp = static_cast<size_t>(nh.W) * kWave;
+  for (int32_t i = 0; i < Nn; ++i)
+    for (int32_t r = 0; r < nh.M; ++r)
+      out[static_cast<size_t>(r) * Nn + i] = countdown_of(nh, ent[static_cast<size_t>(i) * Mp + r], k);
+  return REPLAY_OK;
+}
+
+int32_t replay_get_replica_offsets(replay_ctx *ctx, int32_t *rep_off, int32_t n) {
+  if (!ctx || !rep_off || n < ctx->n_net + 1) return REPLAY_E_ARG;
+  for (int32_t k = 0; k < ctx->n_net; ++k) rep_off[k] = ctx->nets[k].moff;
+  rep_off[ctx->n_net] = ctx->G;
   return REPLAY_OK;
 }
 
@@ -2162,6 +2262,9 @@ int32_t replay_set_uniforms(replay_ctx *ctx, const float *u, int64_t n) {
     ctx->d_uniforms = nullptr;
   }
   if (u) {
+    if (!ctx->uniform)
+      return fail(ctx, REPLAY_E_UNSUPPORTED,
+                  "an injected stream needs every network to have the same replica count");
     const int64_t need = static_cast<int64_t>(ctx->T + 1) * ctx->M * ctx->N;
     if (n != need) return fail(ctx, REPLAY_E_ARG, "uniform stream has the wrong length");
     int rc = dev_alloc(ctx, &ctx->d_uniforms, static_cast<size_t>(n));
@@ -2218,6 +2321,8 @@ int32_t replay_get_slice(replay_ctx *ctx, int32_t k, int32_t *rowptr, int32_t *c
 
 int32_t replay_get_exposure(replay_ctx *ctx, int32_t k, float *out, int64_t n_out) {
   if (!ctx || !out || k < 0 || k >= ctx->T) return REPLAY_E_ARG;
+  if (!ctx->uniform)
+    return fail(ctx, REPLAY_E_UNSUPPORTED, "networks have different replica counts");
   const int64_t n = static_cast<int64_t>(ctx->N) * ctx->M;
   if (n_out < n) return fail(ctx, REPLAY_E_ARG, "exposure buffer too small");
   HIP_OR_FAIL(ctx, hipSetDevice(ctx->p.device));
@@ -2240,6 +2345,8 @@ int32_t replay_get_exposure(replay_ctx *ctx, int32_t k, float *out, int64_t n_ou
 
 int32_t replay_get_exposure_noalloc(replay_ctx *ctx, int32_t k, float *out, int64_t n_out) {
   if (!ctx || !out || k < 0 || k >= ctx->T) return REPLAY_E_ARG;
+  if (!ctx->uniform)
+    return fail(ctx, REPLAY_E_UNSUPPORTED, "networks have different replica counts");
   const int64_t n = static_cast<int64_t>(ctx->N) * ctx->M;
   if (n_out < n) return fail(ctx, REPLAY_E_ARG, "exposure buffer too small");
   if (n > kNoAllocFloats)

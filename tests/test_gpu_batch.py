@@ -182,3 +182,85 @@ def test_random_batches_every_variant(oracle, case):
             states.append(sim.state())
     for s in states[1:]:
         np.testing.assert_array_equal(s, states[0])
+
+
+def _uneven_params(shared: Params, nw: Network) -> Params:
+    return Params(**{**_plain_params(shared, nw).__dict__, "M": nw.M or shared.M})
+
+
+@pytest.mark.parametrize("case", range(10))
+def test_uneven_replica_counts_every_variant(oracle, case):
+    """Batches whose networks have their own replica counts (33, 256 and
+    1000 replicas, i.e. 1, 4 and 16 words per node, in one launch), integer
+    and float windows: under every variant that applies, each network's
+    counts and countdown states equal the oracle's for that network alone
+    with its replica count and stream id.  The bit-sliced variant needs one
+    replica count and is refused."""
+    rng = np.random.default_rng(9100 + case)
+    sizes = [int(x) for x in rng.integers(5, 151, int(rng.integers(2, 6)))]
+    nets = _nets(int(rng.integers(0, 1 << 20)), sizes, long_frac=float(rng.choice([0.0, 0.3, 0.6])))
+    Ms = [33, 256, 1000]
+    rng.shuffle(Ms)
+    for k, nw in enumerate(nets):
+        nw.M = int(Ms[k % 3]) if k < 3 else int(rng.choice([1, 64, 65, 130, 1000]))
+    step, D = [(600.0, 600), (600.0, 1800), (1000.0, 3600), (3600.0, 3600)][int(rng.integers(0, 4))]
+    flags = replay_amd.F_GRAPH if case % 2 else 0
+    shared = Params(M=64, step_size=step, static_network_duration=D, iterations=40,
+                    seed=int(rng.integers(0, 1 << 30)), flags=flags)
+    print(f"case {case}: sizes={sizes} M={[nw.M for nw in nets]} step={step} D={D}", flush=True)
+    expected = []
+    for nw in nets:
+        g = oracle.OracleGraph.from_contacts(nw.n_nodes, nw.src, nw.tgt, nw.start, nw.end)
+        op = _to_oracle_params(oracle, _uneven_params(shared, nw))
+        op.stream = nw.stream
+        expected.append(g.simulate(op, want_state=True)[:2])
+    with Simulation.from_networks(nets, shared) as sim:
+        assert not sim.uniform
+        assert sim.info()["replicas_total"] == sum(nw.M for nw in nets)
+        assert list(np.diff(sim.replica_offsets)) == [nw.M for nw in nets]
+        with pytest.raises(replay_amd.ReplayError):
+            sim.set_variant(replay_amd.V_SLICED)
+        for v in (replay_amd.V_WORD, replay_amd.V_NODE4, replay_amd.V_NODE16):
+            sim.set_variant(v)
+            for rep in range(2 if flags else 1):  # a graph is captured, then replayed
+                sim.reset()
+                sim.run()
+                for k, (counts, state) in enumerate(expected):
+                    np.testing.assert_array_equal(sim.network_counts(k), counts,
+                                                  err_msg=f"variant {v} network {k}")
+                    np.testing.assert_array_equal(sim.network_state(k), state,
+                                                  err_msg=f"variant {v} network {k}")
+
+
+def test_uneven_batch_equals_plain_with_hub_rows():
+    """A hub network (rows beyond the long-row threshold, 130 replicas) with
+    a random network of 1000 replicas in one batch: each equals its own
+    single-network context, counts and states; the batch-wide calls whose
+    layout needs one replica count refuse."""
+    from test_gpu_dense import _two_hubs
+    n, a, b, s, e = _two_hubs(seed=4, leaves=(2600, 4800), T=16)
+    nets = [Network(n_nodes=n, src=a, tgt=b, start=s, end=e, initial_infected=0.03,
+                    infect_prob=0.15, exposed_duration=3600, infectious_duration=5 * 3600,
+                    resistant_duration=10 ** 6, stream=1, M=130)]
+    n2, a2, b2, s2, e2 = _random_case(11, n=300, e=4000, span=16 * 3600, long_frac=0.3)
+    nets.append(Network(n_nodes=n2, src=a2, tgt=b2, start=s2, end=e2, initial_infected=0.1,
+                        infect_prob=0.3, exposed_duration=0, infectious_duration=3 * 3600,
+                        resistant_duration=0, stream=2, M=1000))
+    shared = Params(M=64, step_size=3600.0, static_network_duration=3600, iterations=16, seed=9)
+    with Simulation.from_networks(nets, shared) as sim:
+        sim.run()
+        got = [(sim.network_counts(k), sim.network_state(k)) for k in range(2)]
+        with pytest.raises(ValueError):
+            sim.counts()
+        for call in (sim.state, lambda: sim.exposure(0),
+                     lambda: sim.set_uniforms(np.zeros(17 * 1130 * sim.n_nodes, np.float32))):
+            with pytest.raises(replay_amd.ReplayError) as ei:
+                call()
+            assert ei.value.code == replay_amd.E_UNSUPPORTED
+    for k, nw in enumerate(nets):
+        with Simulation.from_contacts(nw.n_nodes, nw.src, nw.tgt, nw.start, nw.end,
+                                      _uneven_params(shared, nw)) as s1:
+            s1.run()
+            np.testing.assert_array_equal(s1.counts(), got[k][0])
+            np.testing.assert_array_equal(s1.state(), got[k][1])
+    assert got[0][0][-1, :, 1:3].sum() > 0
