@@ -64,6 +64,10 @@ typedef struct replay_params {
                                  copy adds half again); <= 0: 32 GiB */
   uint32_t stream;            /* Philox stream id of this network (counter word 3) */
   int32_t flags;              /* REPLAY_F_* */
+  int64_t max_ring_bytes;     /* device budget for the infection ring (one bit word per
+                                 iteration of the longest timer offset); beyond it the
+                                 compact form keeps one flag bit per word and iteration
+                                 instead.  <= 0: 16 GiB */
 } replay_params;
 
 /* replay_params.flags */
@@ -107,13 +111,16 @@ typedef struct replay_info {
   int32_t steps_done;         /* iterations run since the last reset */
   int32_t truncated_windows;  /* windows whose query hits the cap */
   int64_t step_bytes;         /* algorithmic HBM bytes of one step-kernel launch */
-  int32_t ring_depth;         /* iterations of infection bit words kept on device */
+  int32_t ring_depth;         /* iterations of infection bit words kept on device
+                                 (1 in the compact form) */
   int32_t n_networks;         /* networks in the context (1 unless batched) */
   int32_t variant;            /* step-kernel variant in use (REPLAY_V_*) */
   int32_t graph_ready;        /* a captured full-run hipGraph exists */
   int32_t entry_bytes;        /* bytes per packed entry the step kernels stream:
                                  8 (the 8-byte copy) or 16 */
   int32_t replicas_total;     /* replica columns of all networks (the sum of their M) */
+  int32_t ring_compact;       /* 1: the compact form of the infection ring */
+  int64_t ring_bytes;         /* device bytes of the infection ring (or its flags) */
   float tune_ms[4];           /* device ms of each candidate in the last replay_autotune,
                                  in the order WORD, NODE4, NODE16, SLICED; -1 where the
                                  variant does not apply or no autotune ran */

@@ -44,7 +44,8 @@ class _Params(C.Structure):
                 ("resistant_duration", C.c_int32), ("start_time", C.c_int64),
                 ("static_network_duration", C.c_int64), ("seed", C.c_uint64),
                 ("device", C.c_int32), ("windows_per_epoch", C.c_int32),
-                ("max_slice_bytes", C.c_int64), ("stream", C.c_uint32), ("flags", C.c_int32)]
+                ("max_slice_bytes", C.c_int64), ("stream", C.c_uint32), ("flags", C.c_int32),
+                ("max_ring_bytes", C.c_int64)]
 
 
 class _Network(C.Structure):
@@ -66,7 +67,8 @@ class _Info(C.Structure):
                 ("truncated_windows", C.c_int32), ("step_bytes", C.c_int64),
                 ("ring_depth", C.c_int32), ("n_networks", C.c_int32), ("variant", C.c_int32),
                 ("graph_ready", C.c_int32), ("entry_bytes", C.c_int32),
-                ("replicas_total", C.c_int32), ("tune_ms", C.c_float * 4)]
+                ("replicas_total", C.c_int32), ("ring_compact", C.c_int32),
+                ("ring_bytes", C.c_int64), ("tune_ms", C.c_float * 4)]
 
 
 F_GRAPH = 1
@@ -197,13 +199,14 @@ class Params:
     max_slice_bytes: int = 0
     stream: int = 0
     flags: int = 0
+    max_ring_bytes: int = 0
 
     def _c(self) -> _Params:
         return _Params(self.M, self.step_size, self.iterations, self.initial_infected,
                        self.infect_prob, self.exposed_duration, self.infectious_duration,
                        self.resistant_duration, self.start_time, self.static_network_duration,
                        self.seed, self.device, self.windows_per_epoch, self.max_slice_bytes,
-                       self.stream, self.flags)
+                       self.stream, self.flags, self.max_ring_bytes)
 
 
 @dataclass

@@ -66,6 +66,8 @@ struct StepArgs {
   const uint32_t *rank_limit; // [n_net] of this window, or null (no cap bites)
   const float *p_tab;         // [n_net][tab_max+1]: P of c whole windows, or null
   const int32_t *upd_net;     // network of each update_kernel block (-1: none)
+  uint64_t *F;                // compact ring: [depthF][update waves] flag words, or null
+  int32_t depthF;             // slots of F
   int32_t tab_max;
   int32_t long_row;           // rows with more entries go to long_row_kernel
   uint64_t seed;
@@ -116,6 +118,11 @@ struct replay_ctx {
   int64_t NW = 0;
   bool uniform = true;
   int32_t upd_blocks = 0;  // blocks of update_kernel's grid that hold words
+  // compact ring: the ring keeps one slot (depth 1) and d_F one flag bit per
+  // word and iteration over depthF slots
+  bool compact = false;
+  int32_t depthF = 0;
+  uint64_t *d_F = nullptr;
   // merged plan and slices
   std::vector<int64_t> iter_time;
   std::vector<int32_t> iter_window;

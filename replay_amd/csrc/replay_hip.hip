@@ -945,39 +945,87 @@ constexpr size_t sliced_lds_bytes(int nb) {
              sizeof(int32_t);
 }
 
+// Replicas of word w (entry[w*64 .. w*64+63]) whose last infection was at
+// iteration j (entry value 2j+1): the word X(j) held in the full ring.
+__device__ __forceinline__ uint64_t infected_at(const int32_t *entry, int64_t w, int32_t j) {
+  const int4 *e = reinterpret_cast<const int4 *>(entry + w * kWave);
+  const int32_t v = 2 * j + 1;
+  uint64_t m = 0ull;
+#pragma unroll
+  for (int t = 0; t < kWave / 4; ++t) {
+    const int4 q = e[t];
+    m |= (static_cast<uint64_t>(q.x == v) | (static_cast<uint64_t>(q.y == v) << 1) |
+          (static_cast<uint64_t>(q.z == v) << 2) | (static_cast<uint64_t>(q.w == v) << 3))
+         << (4 * t);
+  }
+  return m;
+}
+
 // Phase 2 of an iteration: one thread per word applies the timer flips for
 // iteration k+1 and advances the running prefix of the infection counts,
 // P[k] = P[k-1] + inf_count[k-1].  Each block's words belong to one network
 // (a.upd_net names it; a network's words start at its own block), so the
 // flip offsets are the block's.
+//
+// The flip at offset n >= 2 needs X(k+1-n), the replicas infected n-1
+// iterations ago.  Full ring: read from slot (k+1-n) mod depth.  Compact
+// form: each wavefront of the grid owns one flag word per iteration (bit l:
+// its lane l's word had an infection), written here from X(k); a flagged
+// word's X(j) is rebuilt from entry[], which still holds 2j+1 for those
+// replicas -- a replica is infected again only once susceptible, and that
+// comes at offset n_S, the last of the three.
 __global__ __launch_bounds__(kBlock) void update_kernel(StepArgs a) {
   const int net = a.upd_net[blockIdx.x];
   if (net >= 0) {
     const NetDev nd = a.nets[net];
     const int64_t j = static_cast<int64_t>(blockIdx.x - nd.blk0) * kBlock + threadIdx.x;
     const int64_t w = nd.woff + j;
-    if (j < nd.nwords) {
-    const uint64_t x = a.X[static_cast<size_t>(a.k % a.depth) * a.NW + w];
-    uint64_t inf = a.I_cur[w];
-    uint64_t sus = a.S[w] & ~x;
-    if (const uint64_t *p = ring_word(a, nd.n_on, w)) inf ^= *p;
-    if (const uint64_t *p = ring_word(a, nd.n_off, w)) inf ^= *p;
-    if (const uint64_t *p = ring_word(a, nd.n_s, w)) sus |= *p;
-    if (nd.n_on == 1) inf ^= x;
-    if (nd.n_off == 1) inf ^= x;
-    if (nd.n_s == 1) sus |= x;
+    const bool valid = j < nd.nwords;
+    const uint64_t x = valid ? a.X[static_cast<size_t>(a.k % a.depth) * a.NW + w] : 0ull;
+    uint64_t inf = valid ? a.I_cur[w] : 0ull;
+    uint64_t sus = valid ? a.S[w] & ~x : 0ull;
     const int32_t k1 = a.k + 1;
-    const bool fl_on = nd.m_on != kNever && nd.m_on >= 1 && k1 == nd.m_on;
-    const bool fl_off = nd.m_off != kNever && nd.m_off >= 1 && k1 == nd.m_off;
-    const bool fl_s = nd.m_s != kNever && nd.m_s >= 1 && k1 == nd.m_s;
-    if (fl_on || fl_off || fl_s) {
-      const uint64_t xi = a.X_init[w];
-      if (fl_on) inf ^= xi;
-      if (fl_off) inf ^= xi;
-      if (fl_s) sus |= xi;
+    if (!a.F) {
+      if (valid) {
+        if (const uint64_t *p = ring_word(a, nd.n_on, w)) inf ^= *p;
+        if (const uint64_t *p = ring_word(a, nd.n_off, w)) inf ^= *p;
+        if (const uint64_t *p = ring_word(a, nd.n_s, w)) sus |= *p;
+      }
+    } else {
+      const int lane = threadIdx.x & (kWave - 1);
+      const size_t wave = static_cast<size_t>(blockIdx.x) * kWavesPerBlock + threadIdx.x / kWave;
+      const size_t n_waves = static_cast<size_t>(gridDim.x) * kWavesPerBlock;
+      const int32_t offs[3] = {nd.n_on, nd.n_off, nd.n_s};
+#pragma unroll
+      for (int t = 0; t < 3; ++t) {
+        const int32_t n = offs[t];
+        if (n == kNever || n < 2 || k1 - n < 0) continue;  // uniform over the block
+        const int32_t jj = k1 - n;
+        const uint64_t f = a.F[static_cast<size_t>(jj % a.depthF) * n_waves + wave];
+        const uint64_t xr = (valid && ((f >> lane) & 1ull)) ? infected_at(a.entry, w, jj) : 0ull;
+        if (t < 2)
+          inf ^= xr;
+        else
+          sus |= xr;
+      }
+      const uint64_t fk = __ballot(x != 0ull);
+      if (lane == 0) a.F[static_cast<size_t>(a.k % a.depthF) * n_waves + wave] = fk;
     }
-    a.I_next[w] = inf;
-    a.S[w] = sus;
+    if (valid) {
+      if (nd.n_on == 1) inf ^= x;
+      if (nd.n_off == 1) inf ^= x;
+      if (nd.n_s == 1) sus |= x;
+      const bool fl_on = nd.m_on != kNever && nd.m_on >= 1 && k1 == nd.m_on;
+      const bool fl_off = nd.m_off != kNever && nd.m_off >= 1 && k1 == nd.m_off;
+      const bool fl_s = nd.m_s != kNever && nd.m_s >= 1 && k1 == nd.m_s;
+      if (fl_on || fl_off || fl_s) {
+        const uint64_t xi = a.X_init[w];
+        if (fl_on) inf ^= xi;
+        if (fl_off) inf ^= xi;
+        if (fl_s) sus |= xi;
+      }
+      a.I_next[w] = inf;
+      a.S[w] = sus;
     }
   }
   const int64_t g = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
@@ -1177,7 +1225,7 @@ void release(replay_ctx *ctx) {
                   ctx->d_X,       ctx->d_Xinit,  ctx->d_entry,  ctx->d_inf,         ctx->d_P,
                   ctx->d_init_count, ctx->d_counts, ctx->d_class_init, ctx->d_segs, ctx->d_nets,
                   ctx->d_net_of,  ctx->d_rank_limit, ctx->d_uniforms, ctx->d_p_tab,
-                  ctx->d_long_rows, ctx->d_upd_net, ctx->d_col_net};
+                  ctx->d_long_rows, ctx->d_upd_net, ctx->d_col_net, ctx->d_F};
   for (void *b : bufs) (void)hipFree(b);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
   ctx->stream = nullptr;
@@ -1270,6 +1318,8 @@ StepArgs rp::step_args(replay_ctx *ctx, int32_t k) {
   a.rank_limit = ctx->any_cut ? ctx->d_rank_limit + static_cast<size_t>(win) * ctx->n_net : nullptr;
   a.p_tab = ctx->d_p_tab;
   a.upd_net = ctx->d_upd_net;
+  a.F = ctx->d_F;
+  a.depthF = ctx->depthF;
   a.tab_max = ctx->tab_max;
   a.long_row = rp::kLongRow;
   a.seed = ctx->p.seed;
@@ -1755,6 +1805,16 @@ int finish_open(replay_ctx *ctx) {
   const size_t NW = static_cast<size_t>(ctx->NW);
   const size_t T = static_cast<size_t>(ctx->T);
   const size_t G = static_cast<size_t>(ctx->G);
+  // The infection ring holds depth slots of every word.  Beyond the budget
+  // the compact form keeps one slot and, over depth slots, one flag bit per
+  // word: a flag word per wavefront of update_kernel's grid.
+  const int64_t ring_budget = p.max_ring_bytes > 0 ? p.max_ring_bytes : (int64_t(16) << 30);
+  const size_t upd_waves = static_cast<size_t>(step_grid(ctx).x) * kWavesPerBlock;
+  if (static_cast<int64_t>(NW) * 8 * ctx->depth > ring_budget && ctx->depth > 1) {
+    ctx->compact = true;
+    ctx->depthF = ctx->depth;
+    ctx->depth = 1;
+  }
   int rc;
   if ((rc = upload(ctx, &ctx->d_entries, reinterpret_cast<const int4 *>(entries.data()),
                    entries.size())) ||
@@ -1771,6 +1831,8 @@ int finish_open(replay_ctx *ctx) {
       (rc = dev_alloc(ctx, &ctx->d_S, NW)) || (rc = dev_alloc(ctx, &ctx->d_I[0], NW)) ||
       (rc = dev_alloc(ctx, &ctx->d_I[1], NW)) ||
       (rc = dev_alloc(ctx, &ctx->d_X, NW * static_cast<size_t>(ctx->depth))) ||
+      (rc = (ctx->compact ? dev_alloc(ctx, &ctx->d_F, upd_waves * static_cast<size_t>(ctx->depthF))
+                          : REPLAY_OK)) ||
       (rc = dev_alloc(ctx, &ctx->d_Xinit, NW)) ||
       (rc = dev_alloc(ctx, &ctx->d_entry, NW * kWave)) ||
       (rc = dev_alloc(ctx, &ctx->d_inf, (T + 1) * G)) ||
@@ -2037,7 +2099,9 @@ int32_t replay_get_info(replay_ctx *ctx, replay_info *out) {
   out->slice_bytes = ctx->n_entries * static_cast<int64_t>(sizeof(rp::Entry)) +
                      static_cast<int64_t>(ctx->epoch_t0.size()) * (ctx->N + 1) * 4;
   const int64_t NW = ctx->NW;
-  out->state_bytes = NW * 8 * (4 + ctx->depth) + NW * kWave * 4;
+  const int64_t flag_bytes =
+      ctx->compact ? static_cast<int64_t>(step_grid(ctx).x) * kWavesPerBlock * 8 * ctx->depthF : 0;
+  out->state_bytes = NW * 8 * (4 + ctx->depth) + NW * kWave * 4 + flag_bytes;
   out->steps_done = ctx->steps_done;
   out->truncated_windows = ctx->truncated;
   // per word: S read + write, own I read, I_next write, X_k write, and one
@@ -2053,10 +2117,18 @@ int32_t replay_get_info(replay_ctx *ctx, replay_info *out) {
   // entries at the width the NODE4 / NODE16 kernels of whole groups stream
   const bool c8 = ctx->d_entries8 && ((ctx->variant == REPLAY_V_NODE4 && ctx->Wg % 4 == 0) ||
                                       (ctx->variant == REPLAY_V_NODE16 && ctx->Wg % 16 == 0));
-  out->step_bytes = NW * 8 * (5 + ring_reads) + static_cast<int64_t>((c8 ? 8.0 : 16.0) * ent) +
+  // compact form: a flag bit per word read per offset and one written, in
+  // place of the ring reads (the entry reads of flagged words depend on the
+  // epidemic and are not counted)
+  const int64_t ring_term = ctx->compact ? (NW / 8) * (ring_reads + 1) : NW * 8 * ring_reads;
+  out->step_bytes = NW * 8 * 5 + ring_term + static_cast<int64_t>((c8 ? 8.0 : 16.0) * ent) +
                     4 * (static_cast<int64_t>(ctx->N) + 1) +
                     12 * static_cast<int64_t>(ctx->G);
   out->ring_depth = ctx->depth;
+  out->ring_compact = ctx->compact ? 1 : 0;
+  out->ring_bytes = ctx->compact ? static_cast<int64_t>(step_grid(ctx).x) * kWavesPerBlock * 8 *
+                                       ctx->depthF
+                                 : NW * 8 * ctx->depth;
   out->n_networks = ctx->n_net;
   out->variant = ctx->variant;
   out->graph_ready = ctx->graph != nullptr;

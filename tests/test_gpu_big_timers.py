@@ -55,15 +55,21 @@ def first_at_least(seq: np.ndarray, upper: int, elapsed: int) -> int:
 
 
 def _run_case(oracle, n, a, b, s, e, p, want_states=True):
+    """The oracle's counts and states, bit for bit, with the full infection
+    ring and with its compact form (forced by a one-byte ring budget)."""
+    from dataclasses import replace
     g = oracle.OracleGraph.from_contacts(n, a, b, s, e)
     oc, ostate, _ = g.simulate(_to_oracle_params(oracle, p, threads=8), want_state=want_states)
-    with Simulation.from_contacts(n, a, b, s, e, p) as sim:
-        assert sim.info()["ring_depth"] >= 1
-        sim.run()
-        counts, state = sim.counts(), sim.state()
-    np.testing.assert_array_equal(counts, oc)
-    if want_states:
-        np.testing.assert_array_equal(state.reshape(-1), ostate)
+    for budget, compact in ((0, 0), (1, 1)):
+        with Simulation.from_contacts(n, a, b, s, e, replace(p, max_ring_bytes=budget)) as sim:
+            info = sim.info()
+            assert info["ring_compact"] == compact
+            assert info["ring_depth"] >= 1
+            sim.run()
+            counts, state = sim.counts(), sim.state()
+        np.testing.assert_array_equal(counts, oc, err_msg=f"compact={compact}")
+        if want_states:
+            np.testing.assert_array_equal(state.reshape(-1), ostate, err_msg=f"compact={compact}")
     return counts
 
 
@@ -153,3 +159,75 @@ def test_hub_rows_with_big_timers(oracle):
             np.testing.assert_array_equal(sim.counts(), oc, err_msg=f"variant {v}")
             np.testing.assert_array_equal(sim.state(), ostate, err_msg=f"variant {v}")
         assert oc[-1, :, 1:3].sum() > 0
+
+
+@pytest.mark.parametrize("case", range(12))
+def test_compact_ring_random_matches_oracle(oracle, case):
+    """Random graphs and timer settings (SEIRS with all three flip offsets
+    inside the horizon, SIS, permanent R) with the compact ring forced:
+    counts and states equal the oracle's under every variant."""
+    rng = np.random.default_rng(4400 + case)
+    n = int(rng.choice([20, 70, 150]))
+    M = int(rng.choice([1, 64, 130, 256, 1024]))
+    step, D = [(600.0, 600), (600.0, 1800), (1000.0, 3600)][int(rng.integers(0, 3))]
+    Ed, Id, Rd = [(1200, 5400, 3600), (0, 7200, 0), (1800, 3000, 10 ** 7),
+                  (600, 1800, 1200)][int(rng.integers(0, 4))]
+    nn, a, b, s, e = _random_case(int(rng.integers(0, 1 << 30)), n=n, e=n * 20, long_frac=0.3)
+    p = Params(M=M, step_size=step, static_network_duration=D, iterations=60,
+               initial_infected=0.1, infect_prob=float(rng.choice([0.1, 0.4])),
+               exposed_duration=Ed, infectious_duration=Id, resistant_duration=Rd,
+               seed=int(rng.integers(0, 1000)), max_ring_bytes=1)
+    print(f"case {case}: n={n} M={M} step={step} D={D} E,I,R={Ed},{Id},{Rd}", flush=True)
+    g = oracle.OracleGraph.from_contacts(nn, a, b, s, e)
+    oc, ostate, _ = g.simulate(_to_oracle_params(oracle, p, threads=8), want_state=True)
+    with Simulation.from_contacts(nn, a, b, s, e, p) as sim:
+        assert sim.info()["ring_compact"] == (1 if sim.info()["ring_depth"] == 1 else 0)
+        for v in (replay_amd.V_WORD, replay_amd.V_NODE4, replay_amd.V_NODE16, replay_amd.V_SLICED):
+            sim.set_variant(v)
+            sim.reset()
+            sim.run()
+            np.testing.assert_array_equal(sim.counts(), oc, err_msg=f"variant {v}")
+            np.testing.assert_array_equal(sim.state(), ostate, err_msg=f"variant {v}")
+
+
+def test_ring_beyond_device_memory_runs_compact():
+    """50,000 nodes x 4096 replicas at the reference's RollerNet settings
+    (30 s steps, 7-day infectious period, R = 99,999,999 s) over 20,400
+    iterations: the infectious period ends about 18,900 iterations after an
+    infection, so a full ring would hold that many slots of 25.6 MB, about
+    484 GB, more than the 288 GB of the device.  The context opens in the
+    compact form by default and runs; replicas 0-63 equal a 64-replica run
+    of the same network with the full ring (a replica's draws do not depend
+    on M)."""
+    Ed, Id, Rd, step = 0, 604800, 99_999_999, 30.0
+    seq = countdown_sequence(Ed + Id + Rd, step, 25000)
+    n_off = first_at_least(seq, Ed + Id + Rd, Ed + Id)
+    T = n_off + 1500
+    n, M = 50_000, 4096
+    depth_bytes = n * (M // 64) * 8 * n_off
+    assert depth_bytes > 288e9, depth_bytes
+    rng = np.random.default_rng(77)
+    k = 400_000
+    a = rng.integers(0, n, k)
+    b = (a + 1 + rng.integers(0, n - 1, k)) % n
+    s = 946713600 + rng.integers(0, int(T * step), k)
+    e = s + np.minimum(rng.exponential(3600, k).astype(np.int64) + 1, 8 * 3600)
+    p = Params(M=M, step_size=step, static_network_duration=30, iterations=T,
+               initial_infected=2e-4, infect_prob=0.3, exposed_duration=Ed,
+               infectious_duration=Id, resistant_duration=Rd, seed=11)
+    args = (n, a.astype(np.int32), b.astype(np.int32), s.astype(np.int64), e.astype(np.int64))
+    with Simulation.from_contacts(*args, p) as sim:
+        info = sim.info()
+        print(f"T={T} ring_compact={info['ring_compact']} ring_bytes={info['ring_bytes']}",
+              flush=True)
+        assert info["ring_compact"] == 1 and info["ring_bytes"] < 16e9
+        sim.run()
+        counts = sim.counts()
+    from dataclasses import replace
+    with Simulation.from_contacts(*args, replace(p, M=64)) as small:
+        assert small.info()["ring_compact"] == 0
+        small.run()
+        c64 = small.counts()
+    np.testing.assert_array_equal(counts[:, :64], c64)
+    assert (counts.sum(axis=2) == n).all()
+    assert counts[-1, :, 2].sum() > 0  # some replicas reached R within the horizon
