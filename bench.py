@@ -46,28 +46,51 @@ def log(msg: str) -> None:
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
+def cpu_model() -> str:
+    """The host CPU's model name and logical CPU count, from /proc/cpuinfo."""
+    try:
+        names = [ln.split(":", 1)[1].strip() for ln in open("/proc/cpuinfo")
+                 if ln.startswith("model name")]
+        return f"{names[0]} ({len(names)} logical CPUs visible)" if names else "unknown"
+    except OSError:
+        return "unknown"
+
+
 def cpu_baseline(name, n_nodes, src, tgt, start, end, p, threads: int, M_s: int, T_s: int,
                  note: str = "") -> dict:
-    """Oracle on the host: same network, M and T truncated to a bounded sample."""
+    """Oracle on the host: same network, M and T truncated to a bounded
+    sample, with `threads` OpenMP threads (the reference's default is every
+    core, cu:716-723) and again with one thread (SURVEY's baseline plan asks
+    for both, with the CPU model)."""
     from oracle import oracle_py
     if not os.path.exists(oracle_py.LIB_PATH):
         oracle_py.build()
     M_s = min(p.M, M_s)
     T_s = min(p.iterations, T_s)
     g = oracle_py.OracleGraph.from_contacts(n_nodes, src, tgt, start, end)
-    op = oracle_py.make_params(M=M_s, step_size=p.step_size, iterations=T_s,
-                               initial_infected=p.initial_infected, infect_prob=p.infect_prob,
-                               exposed_duration=p.exposed_duration,
-                               infectious_duration=p.infectious_duration,
-                               resistant_duration=p.resistant_duration, start_time=p.start_time,
-                               static_network_duration=p.static_network_duration, seed=p.seed,
-                               threads=threads, use_libm_exp=1)
-    _, _, secs = g.simulate(op)
-    return {"value": n_nodes * M_s * T_s / secs, "unit": "replica*node*steps/s",
+
+    def timed(nt):
+        op = oracle_py.make_params(M=M_s, step_size=p.step_size, iterations=T_s,
+                                   initial_infected=p.initial_infected,
+                                   infect_prob=p.infect_prob,
+                                   exposed_duration=p.exposed_duration,
+                                   infectious_duration=p.infectious_duration,
+                                   resistant_duration=p.resistant_duration,
+                                   start_time=p.start_time,
+                                   static_network_duration=p.static_network_duration,
+                                   seed=p.seed, threads=nt, use_libm_exp=1)
+        return g.simulate(op)[2]
+
+    secs = timed(threads)
+    secs1 = timed(1)
+    work = n_nodes * M_s * T_s
+    return {"value": work / secs, "unit": "replica*node*steps/s",
             "cores": threads, "kind": "port",
+            "value_1_thread": work / secs1,
+            "cpu_model": cpu_model(),
             "sample": f"{name}: {note}N={n_nodes}, M={M_s} of {p.M}, first {T_s} of "
-                      f"{p.iterations} iterations; loop {secs:.2f} s (glibc expf, OpenMP rows "
-                      f"when nnz > 20000)"}
+                      f"{p.iterations} iterations; loop {secs:.2f} s on {threads} threads, "
+                      f"{secs1:.2f} s on 1 (glibc expf, OpenMP rows when nnz > 20000)"}
 
 
 def measured_traffic(workload: str):
@@ -81,16 +104,16 @@ def measured_traffic(workload: str):
         return json.load(f)
 
 
-def mean_nnz(sim, T: int):
+def mean_nnz(sim, T: int, samples: int = 64):
     """Mean nonzeros of the window CSR over the run's iterations (every
-    iteration, or 64 evenly spaced ones when there are more than 256),
+    iteration, or `samples` evenly spaced ones when there are more than 256),
     counted on the device; and which iterations were counted.  The count
     walks each row with one thread, which takes seconds per window on hub
     rows, hence the sample."""
     if T <= 256:
         ks = list(range(T))
     else:
-        ks = sorted({round(i * (T - 1) / 63) for i in range(64)})
+        ks = sorted({round(i * (T - 1) / (samples - 1)) for i in range(samples)})
     nnz = sum(sim.slice_nnz(k) for k in ks) / len(ks)
     return nnz, ("every iteration" if T <= 256 else f"{len(ks)} evenly spaced iterations")
 
@@ -184,6 +207,45 @@ def bench_single(args, w) -> dict:
         out["cpu_baseline"] = cpu_baseline(w.name, w.n_nodes, w.src, w.tgt, w.start, w.end,
                                            w.params, min(16, os.cpu_count() or 1), M_s, T_s)
     return out
+
+
+def dense_record(args) -> dict:
+    """The high-density workload (50k nodes, 40M contacts, 500 power-law
+    hubs, 4096 replicas, 4032 iterations of 5 min) timed like the headline
+    one: reset + all iterations replayed from one hipGraph, inputs resident,
+    after an untimed warm-up; a sub-record of the default line."""
+    import replay_amd
+    from replay_amd import Params, Simulation, workloads
+    t0 = time.time()
+    w = workloads.config2_dense()
+    p = Params(**{**w.params.__dict__, "flags": replay_amd.F_GRAPH})
+    sim = Simulation.from_contacts(w.n_nodes, w.src, w.tgt, w.start, w.end, p)
+    variant = sim.autotune(48)
+    info = sim.info()
+    N, M, T = info["n_nodes"], info["M"], w.params.iterations
+    nnz, nnz_note = mean_nnz(sim, T, samples=16)
+    log(f"{w.name}: N={N} M={M} T={T} contacts={info['n_contacts']} variant={variant} "
+        f"ring_form={info['ring_form']} state {info['state_bytes'] / 1e9:.2f} GB, "
+        f"ready in {time.time() - t0:.1f}s")
+    steps = max(1, min(args.steps, 2))
+    wall, kernel_ms = time_runs(sim, T, steps, 1)
+    counts = sim.counts()
+    sim.close()
+    per_launch = (sum(kernel_ms) / len(kernel_ms)) / 1e3 / T
+    rf = roofline(info, per_launch, w.name, nnz, nnz_note)
+    log(f"dense: {wall * 1e3 / steps:.1f} ms per run, {N * M * T * steps / wall:.3e} r*n*s/s")
+    return {"workload": w.name, "value": N * M * T * steps / wall,
+            "unit": "replica*node*steps/s", "steps": steps, "warmup": 1,
+            "ms_per_step": wall * 1e3 / steps, "nodes": N, "replicas": M, "iterations": T,
+            "contacts": info["n_contacts"], "variant": variant, "hipgraph": True,
+            "ring_form": ["offsets", "compact", "history"][info["ring_form"]],
+            "state_bytes": info["state_bytes"],
+            "final_recovered_mean": float(counts[-1, :, 2].mean()),
+            "roofline": {k: rf[k] for k in ("bound", "achieved", "peak", "unit", "frac",
+                                            "bytes_per_launch", "us_per_launch",
+                                            "nnz_per_iteration")},
+            "roofline_note": f"bytes per iteration by the fused SpMM definition of the main "
+                             f"line; window nonzeros from {nnz_note}"}
 
 
 def bench_batched(args, w) -> dict:
@@ -366,6 +428,8 @@ def main() -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-plain", action="store_true",
                     help="batched workload only: skip the plain per-network loop")
+    ap.add_argument("--no-dense", action="store_true",
+                    help="batched workload only: skip the high-density sub-record")
     ap.add_argument("--eager", action="store_true",
                     help="launch every step eagerly (no hipGraph), e.g. under rocprofv3 --pmc")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
@@ -410,6 +474,8 @@ def main() -> int:
            "ms_per_step": res["ms_per_step"], "higher_is_better": True, "scaling": "weak",
            "vs_baseline": None, "dtype": "f32", "data": "synthetic", "config": res["config"],
            "roofline": res["roofline"], "cpu_baseline": res.get("cpu_baseline")}
+    if args.workload == "config3" and not args.no_dense:
+        out["dense_workload"] = dense_record(args)
     print(json.dumps(out))
     return 0
 

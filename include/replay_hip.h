@@ -75,6 +75,8 @@ typedef struct replay_params {
 #define REPLAY_F_AUTOTUNE 2   /* time the kernel variants at open and keep the fastest */
 #define REPLAY_F_WIDE_ENTRIES 4 /* step kernels read the 16-byte packed entries even where
                                    their 8-byte form fits (comparisons and measurement) */
+#define REPLAY_F_NO_HISTORY 8 /* keep only the infection ring's flip offsets even where every
+                                 iteration would fit (comparisons and measurement) */
 
 /* One network of a batch (replay_open_batch): its contacts, as in
    replay_open_contacts, its own epidemiological parameters and its own
@@ -119,12 +121,18 @@ typedef struct replay_info {
   int32_t entry_bytes;        /* bytes per packed entry the step kernels stream:
                                  8 (the 8-byte copy) or 16 */
   int32_t replicas_total;     /* replica columns of all networks (the sum of their M) */
-  int32_t ring_compact;       /* 1: the compact form of the infection ring */
+  int32_t ring_form;          /* REPLAY_RING_*: how the infection ring is kept */
   int64_t ring_bytes;         /* device bytes of the infection ring (or its flags) */
   float tune_ms[4];           /* device ms of each candidate in the last replay_autotune,
                                  in the order WORD, NODE4, NODE16, SLICED; -1 where the
                                  variant does not apply or no autotune ran */
 } replay_info;
+
+/* replay_info.ring_form */
+#define REPLAY_RING_OFFSETS 0 /* slots back to the longest timer offset; each infection
+                                 recorded per replica as it is drawn */
+#define REPLAY_RING_COMPACT 1 /* one slot plus a flag bit per word and iteration */
+#define REPLAY_RING_HISTORY 2 /* every iteration; per-replica records derived when read */
 
 /* replay_info.variant */
 #define REPLAY_V_WORD 0       /* one pass over a node's contacts per 64-replica word */

@@ -67,13 +67,15 @@ class _Info(C.Structure):
                 ("truncated_windows", C.c_int32), ("step_bytes", C.c_int64),
                 ("ring_depth", C.c_int32), ("n_networks", C.c_int32), ("variant", C.c_int32),
                 ("graph_ready", C.c_int32), ("entry_bytes", C.c_int32),
-                ("replicas_total", C.c_int32), ("ring_compact", C.c_int32),
+                ("replicas_total", C.c_int32), ("ring_form", C.c_int32),
                 ("ring_bytes", C.c_int64), ("tune_ms", C.c_float * 4)]
 
 
 F_GRAPH = 1
 F_AUTOTUNE = 2
 F_WIDE_ENTRIES = 4
+F_NO_HISTORY = 8
+RING_OFFSETS, RING_COMPACT, RING_HISTORY = 0, 1, 2
 V_WORD, V_NODE4, V_NODE16, V_SLICED = 0, 4, 16, 1
 
 

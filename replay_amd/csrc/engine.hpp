@@ -68,6 +68,9 @@ struct StepArgs {
   const int32_t *upd_net;     // network of each update_kernel block (-1: none)
   uint64_t *F;                // compact ring: [depthF][update waves] flag words, or null
   int32_t depthF;             // slots of F
+  int32_t record;             // 1: each infection is written to entry[] and counted in
+                              // inf_count as it is drawn; 0 (the ring keeps every
+                              // iteration): both are derived from the ring when read
   int32_t tab_max;
   int32_t long_row;           // rows with more entries go to long_row_kernel
   uint64_t seed;
@@ -123,6 +126,14 @@ struct replay_ctx {
   bool compact = false;
   int32_t depthF = 0;
   uint64_t *d_F = nullptr;
+  // history ring: X keeps every iteration (depth T + 1), the step kernels
+  // record nothing per infection, and inf_count / entry are derived from X
+  // when counts or states are read; tallied = iterations whose inf_count is
+  // derived
+  bool history = false;
+  int32_t tallied = 0;
+  int32_t rebuilt = 0;  // iterations whose infections entry[] reflects (history ring)
+  int32_t *d_tally = nullptr;  // (network, word) pairs of the tally grid, as net << 16 | q
   // merged plan and slices
   std::vector<int64_t> iter_time;
   std::vector<int32_t> iter_window;

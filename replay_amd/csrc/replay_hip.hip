@@ -510,7 +510,7 @@ __device__ __forceinline__ uint64_t draw_word(const StepArgs &a, const NetDev &n
     else
       infect = 1.0f - rp::rp_expf(nd.log_base * (E / 3600.0f)) > u;
   }
-  if (infect) {
+  if (infect && a.record) {
     // entry is laid out [word][64 replicas]; wi is the node's first word
     a.entry[(static_cast<size_t>(wi) + Q) * kWave + lane] = 2 * a.k + 1;
     atomicAdd(a.inf_count + static_cast<size_t>(a.k) * a.G + nd.moff + r, 1);
@@ -924,7 +924,7 @@ __global__ __launch_bounds__(kBlock) void sliced_kernel(StepArgs a, int G) {
           const float u = kInjected ? a.uniforms[static_cast<size_t>(r) * a.N + i] : rp::u24(pw[t]);
           infect = P > u;
         }
-        if (infect) {
+        if (infect && a.record) {
           // every network has a.M replicas here: node i's words start at i * W
           a.entry[static_cast<size_t>(i) * W * kWave + r] = 2 * a.k + 1;
           atomicAdd(a.inf_count + static_cast<size_t>(a.k) * a.G + static_cast<size_t>(net) * a.M + r,
@@ -1030,9 +1030,75 @@ __global__ __launch_bounds__(kBlock) void update_kernel(StepArgs a) {
   }
   const int64_t g = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
   const int64_t G = a.G;
-  if (a.k >= 1 && g < G)
+  if (a.record && a.k >= 1 && g < G)
     a.P[static_cast<size_t>(a.k) * G + g] =
         a.P[static_cast<size_t>(a.k - 1) * G + g] + a.inf_count[static_cast<size_t>(a.k - 1) * G + g];
+}
+
+// History ring: infections of iterations j0 + blockIdx.y per replica.  A
+// block takes one (network, word q) pair (tally[blockIdx.x] = net << 16 | q);
+// its 4 wavefronts walk the network's nodes 64 at a time (lane = node), and
+// the column count of the 64 gathered X words gives each lane the number of
+// those nodes whose replica 64 q + lane was infected.  The wavefronts' sums
+// meet in LDS and lane r writes inf_count of replica column moff + 64 q + r:
+// every column of the iteration is written, no atomics.
+__global__ __launch_bounds__(kBlock) void tally_kernel(StepArgs a, const int32_t *tally,
+                                                       int32_t j0) {
+  __shared__ uint32_t part[kWavesPerBlock][kWave];
+  const int lane = threadIdx.x & (kWave - 1);
+  const int wv = threadIdx.x / kWave;
+  const int32_t t = tally[blockIdx.x];
+  const int net = t >> 16, q = t & 0xFFFF;
+  const NetDev nd = a.nets[net];
+  const int32_t j = j0 + static_cast<int32_t>(blockIdx.y);
+  const uint64_t *X = a.X + static_cast<size_t>(j % a.depth) * a.NW;
+  const int32_t n_nodes = static_cast<int32_t>(nd.nwords / nd.W);
+  uint32_t acc = 0;
+  for (int32_t c0 = wv * kWave; c0 < n_nodes; c0 += 4 * kBlock) {
+    // up to four 64-node chunks per pass, their loads in flight together
+    uint64_t x[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int32_t node = c0 + u * kBlock + lane;
+      x[u] = node < n_nodes ? X[nd.woff + static_cast<int64_t>(node) * nd.W + q] : 0ull;
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+      if (__ballot(x[u] != 0ull) != 0ull) acc += column_count(x[u], lane);
+  }
+  part[wv][lane] = acc;
+  __syncthreads();
+  if (wv == 0) {
+    const uint32_t sum = part[0][lane] + part[1][lane] + part[2][lane] + part[3][lane];
+    const int r = q * kWave + lane;
+    if (r < nd.M) a.inf_count[static_cast<size_t>(j) * a.G + nd.moff + r] = static_cast<int32_t>(sum);
+  }
+}
+
+// Running prefix P[j] = P[j-1] + inf_count[j-1] for j = j0+1 .. j1, one
+// thread per replica column.
+__global__ void prefix_scan_kernel(int32_t *P, const int32_t *inf_count, int32_t G, int32_t j0,
+                                   int32_t j1) {
+  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= G) return;
+  int32_t p = P[static_cast<size_t>(j0) * G + g];
+  for (int32_t j = j0 + 1; j <= j1; ++j) {
+    p += inf_count[static_cast<size_t>(j - 1) * G + g];
+    P[static_cast<size_t>(j) * G + g] = p;
+  }
+}
+
+// History ring: brings entry[] (each replica's last infection, 2j+1; 0
+// seeded, -1 never, as init_kernel leaves it) from iteration j0 up to k by
+// walking X forward over j0 .. k-1: a later infection overwrites an earlier
+// one.  One thread per word.
+__global__ void rebuild_entry_kernel(StepArgs a, int32_t j0, int32_t k) {
+  const int64_t w = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (w >= a.NW) return;
+  int32_t *e = a.entry + w * kWave;
+  for (int32_t j = j0; j < k; ++j)
+    for (uint64_t x = a.X[static_cast<size_t>(j % a.depth) * a.NW + w]; x != 0ull; x &= x - 1)
+      e[__builtin_ctzll(x)] = 2 * j + 1;
 }
 
 struct InitArgs {
@@ -1225,7 +1291,7 @@ void release(replay_ctx *ctx) {
                   ctx->d_X,       ctx->d_Xinit,  ctx->d_entry,  ctx->d_inf,         ctx->d_P,
                   ctx->d_init_count, ctx->d_counts, ctx->d_class_init, ctx->d_segs, ctx->d_nets,
                   ctx->d_net_of,  ctx->d_rank_limit, ctx->d_uniforms, ctx->d_p_tab,
-                  ctx->d_long_rows, ctx->d_upd_net, ctx->d_col_net, ctx->d_F};
+                  ctx->d_long_rows, ctx->d_upd_net, ctx->d_col_net, ctx->d_F, ctx->d_tally};
   for (void *b : bufs) (void)hipFree(b);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
   ctx->stream = nullptr;
@@ -1320,6 +1386,7 @@ StepArgs rp::step_args(replay_ctx *ctx, int32_t k) {
   a.upd_net = ctx->d_upd_net;
   a.F = ctx->d_F;
   a.depthF = ctx->depthF;
+  a.record = ctx->history ? 0 : 1;
   a.tab_max = ctx->tab_max;
   a.long_row = rp::kLongRow;
   a.seed = ctx->p.seed;
@@ -1522,6 +1589,8 @@ int do_reset(replay_ctx *ctx) {
   ctx->cur = 0;
   ctx->steps_done = 0;
   ctx->counts_done = -1;
+  ctx->tallied = 0;
+  ctx->rebuilt = 0;
   return REPLAY_OK;
 }
 
@@ -1531,8 +1600,25 @@ int enqueue_counts(replay_ctx *ctx) {
   if (j0 > j1) return REPLAY_OK;
   const int G = ctx->G;
   const int tb = 256, nb = (G + tb - 1) / tb;
-  hipLaunchKernelGGL(prefix_tail_kernel, dim3(nb), dim3(tb), 0, ctx->stream, ctx->d_P, ctx->d_inf,
-                     G, j1);
+  if (ctx->history) {
+    // infections of the iterations not yet tallied, from the ring, and the
+    // running prefix over them
+    const int32_t t0 = ctx->tallied;
+    if (j1 > t0) {
+      StepArgs a = step_args(ctx, 0);
+      const dim3 grid(static_cast<unsigned>(std::accumulate(
+                          ctx->nets.begin(), ctx->nets.end(), 0,
+                          [](int acc, const NetHost &nh) { return acc + nh.W; })),
+                      static_cast<unsigned>(j1 - t0));
+      hipLaunchKernelGGL(tally_kernel, grid, dim3(kBlock), 0, ctx->stream, a, ctx->d_tally, t0);
+      hipLaunchKernelGGL(prefix_scan_kernel, dim3(nb), dim3(tb), 0, ctx->stream, ctx->d_P,
+                         ctx->d_inf, G, t0, j1);
+      ctx->tallied = j1;
+    }
+  } else {
+    hipLaunchKernelGGL(prefix_tail_kernel, dim3(nb), dim3(tb), 0, ctx->stream, ctx->d_P,
+                       ctx->d_inf, G, j1);
+  }
   hipLaunchKernelGGL(counts_kernel, dim3(nb, j1 - j0 + 1), dim3(tb), 0, ctx->stream,
                      ctx->d_counts, ctx->d_P, ctx->d_init_count, ctx->d_class_init, ctx->d_segs,
                      ctx->d_col_net, G, ctx->T, j0, j1);
@@ -1808,9 +1894,20 @@ int finish_open(replay_ctx *ctx) {
   // The infection ring holds depth slots of every word.  Beyond the budget
   // the compact form keeps one slot and, over depth slots, one flag bit per
   // word: a flag word per wavefront of update_kernel's grid.
+  // Where every iteration fits (T + 1 slots), the ring keeps them all and
+  // the step kernels record nothing per infection (history form).
   const int64_t ring_budget = p.max_ring_bytes > 0 ? p.max_ring_bytes : (int64_t(16) << 30);
   const size_t upd_waves = static_cast<size_t>(step_grid(ctx).x) * kWavesPerBlock;
-  if (static_cast<int64_t>(NW) * 8 * ctx->depth > ring_budget && ctx->depth > 1) {
+  std::vector<int32_t> tally;
+  if (ctx->T >= 1 && !(p.flags & REPLAY_F_NO_HISTORY) &&
+      static_cast<int64_t>(NW) * 8 * (ctx->T + 1) <= ring_budget) {
+    ctx->history = true;
+    ctx->depth = ctx->T + 1;
+    for (int32_t n = 0; n < ctx->n_net; ++n)
+      for (int32_t q = 0; q < ctx->nets[n].W; ++q) tally.push_back(n << 16 | q);
+    if (ctx->n_net >= (1 << 15) || ctx->W >= (1 << 16))
+      return fail(ctx, REPLAY_E_UNSUPPORTED, "too many networks or words for the history tally");
+  } else if (static_cast<int64_t>(NW) * 8 * ctx->depth > ring_budget && ctx->depth > 1) {
     ctx->compact = true;
     ctx->depthF = ctx->depth;
     ctx->depth = 1;
@@ -1833,6 +1930,7 @@ int finish_open(replay_ctx *ctx) {
       (rc = dev_alloc(ctx, &ctx->d_X, NW * static_cast<size_t>(ctx->depth))) ||
       (rc = (ctx->compact ? dev_alloc(ctx, &ctx->d_F, upd_waves * static_cast<size_t>(ctx->depthF))
                           : REPLAY_OK)) ||
+      (rc = (ctx->history ? upload(ctx, &ctx->d_tally, tally.data(), tally.size()) : REPLAY_OK)) ||
       (rc = dev_alloc(ctx, &ctx->d_Xinit, NW)) ||
       (rc = dev_alloc(ctx, &ctx->d_entry, NW * kWave)) ||
       (rc = dev_alloc(ctx, &ctx->d_inf, (T + 1) * G)) ||
@@ -1940,6 +2038,19 @@ int noalloc_scratch(replay_ctx *ctx, float **ptr) {
   return REPLAY_OK;
 }
 
+// History ring: derive entry[] (each replica's last infection) from the
+// ring before a state read.
+int rebuild_entries(replay_ctx *ctx) {
+  if (!ctx->history || ctx->rebuilt >= ctx->steps_done) return REPLAY_OK;
+  StepArgs a = step_args(ctx, 0);
+  const int64_t nb = (ctx->NW + kBlock - 1) / kBlock;
+  hipLaunchKernelGGL(rebuild_entry_kernel, dim3(static_cast<unsigned>(nb)), dim3(kBlock), 0,
+                     ctx->stream, a, ctx->rebuilt, ctx->steps_done);
+  HIP_OR_FAIL(ctx, hipGetLastError());
+  ctx->rebuilt = ctx->steps_done;
+  return REPLAY_OK;
+}
+
 // Countdown of a replica after k iterations from its entry value (-1 never
 // infected, 0 initial seed, 2j+1 infected at iteration j), by the network's
 // tabulated sequences.
@@ -1952,10 +2063,12 @@ int32_t countdown_of(const NetHost &nh, int32_t s, int32_t k) {
 // Captures the step and counts kernels of a full run (iterations 0..T-1
 // from a reset state) into one executable graph.
 int build_graph(replay_ctx *ctx) {
-  const int32_t saved_cur = ctx->cur, saved_done = ctx->steps_done, saved_counts = ctx->counts_done;
+  const int32_t saved_cur = ctx->cur, saved_done = ctx->steps_done, saved_counts = ctx->counts_done,
+                saved_tallied = ctx->tallied;
   ctx->cur = 0;
   ctx->steps_done = 0;
   ctx->counts_done = -1;
+  ctx->tallied = 0;
   hipGraph_t g = nullptr;
   HIP_OR_FAIL(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
   int rc = enqueue_steps(ctx, ctx->T);
@@ -1964,6 +2077,7 @@ int build_graph(replay_ctx *ctx) {
   ctx->cur = saved_cur;
   ctx->steps_done = saved_done;
   ctx->counts_done = saved_counts;
+  ctx->tallied = saved_tallied;
   if (rc != REPLAY_OK) {
     if (g) (void)hipGraphDestroy(g);
     return rc;
@@ -2125,7 +2239,9 @@ int32_t replay_get_info(replay_ctx *ctx, replay_info *out) {
                     4 * (static_cast<int64_t>(ctx->N) + 1) +
                     12 * static_cast<int64_t>(ctx->G);
   out->ring_depth = ctx->depth;
-  out->ring_compact = ctx->compact ? 1 : 0;
+  out->ring_form = ctx->history   ? REPLAY_RING_HISTORY
+                  : ctx->compact ? REPLAY_RING_COMPACT
+                                 : REPLAY_RING_OFFSETS;
   out->ring_bytes = ctx->compact ? static_cast<int64_t>(step_grid(ctx).x) * kWavesPerBlock * 8 *
                                        ctx->depthF
                                  : NW * 8 * ctx->depth;
@@ -2158,6 +2274,7 @@ int32_t replay_run(replay_ctx *ctx, int32_t steps) {
     ctx->steps_done = ctx->T;
     ctx->cur = ctx->T & 1;
     ctx->counts_done = ctx->T;
+    ctx->tallied = ctx->T;
     return REPLAY_OK;
   }
   int rc = enqueue_steps(ctx, steps);
@@ -2274,6 +2391,7 @@ int32_t replay_get_state(replay_ctx *ctx, int32_t *out, int64_t n_out) {
   const int64_t n = static_cast<int64_t>(ctx->N) * ctx->M;
   if (n_out < n) return fail(ctx, REPLAY_E_ARG, "state buffer too small");
   HIP_OR_FAIL(ctx, hipSetDevice(ctx->p.device));
+  if (int rc = rebuild_entries(ctx)) return rc;
   // entry is [word][64 replicas]; with one replica count node i's words
   // start at i * W, so this is [node][W * 64]
   std::vector<int32_t> ent(static_cast<size_t>(ctx->NW) * kWave);
@@ -2298,6 +2416,7 @@ int32_t replay_get_network_state(replay_ctx *ctx, int32_t net, int32_t *out, int
   const int64_t n = static_cast<int64_t>(Nn) * nh.M;
   if (n_out < n) return fail(ctx, REPLAY_E_ARG, "state buffer too small");
   HIP_OR_FAIL(ctx, hipSetDevice(ctx->p.device));
+  if (int rc = rebuild_entries(ctx)) return rc;
   // the network's words are one run of Nn * W words from woff
   std::vector<int32_t> ent(static_cast<size_t>(Nn) * nh.W * kWave);
   HIP_OR_FAIL(ctx, hipMemcpyAsync(ent.data(), ctx->d_entry + static_cast<size_t>(nh.woff) * kWave,

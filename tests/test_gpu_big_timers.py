@@ -55,21 +55,26 @@ def first_at_least(seq: np.ndarray, upper: int, elapsed: int) -> int:
 
 
 def _run_case(oracle, n, a, b, s, e, p, want_states=True):
-    """The oracle's counts and states, bit for bit, with the full infection
-    ring and with its compact form (forced by a one-byte ring budget)."""
+    """The oracle's counts and states, bit for bit, with every form of the
+    infection ring: every iteration kept (the default here), slots back to
+    the longest timer offset (REPLAY_F_NO_HISTORY), and the compact form
+    (forced by a one-byte ring budget)."""
     from dataclasses import replace
     g = oracle.OracleGraph.from_contacts(n, a, b, s, e)
     oc, ostate, _ = g.simulate(_to_oracle_params(oracle, p, threads=8), want_state=want_states)
-    for budget, compact in ((0, 0), (1, 1)):
-        with Simulation.from_contacts(n, a, b, s, e, replace(p, max_ring_bytes=budget)) as sim:
+    forms = ((dict(), replay_amd.RING_HISTORY),
+             (dict(flags=replay_amd.F_NO_HISTORY), replay_amd.RING_OFFSETS),
+             (dict(max_ring_bytes=1), replay_amd.RING_COMPACT))
+    for kw, form in forms:
+        with Simulation.from_contacts(n, a, b, s, e, replace(p, **kw)) as sim:
             info = sim.info()
-            assert info["ring_compact"] == compact
+            assert info["ring_form"] == form
             assert info["ring_depth"] >= 1
             sim.run()
             counts, state = sim.counts(), sim.state()
-        np.testing.assert_array_equal(counts, oc, err_msg=f"compact={compact}")
+        np.testing.assert_array_equal(counts, oc, err_msg=f"ring form {form}")
         if want_states:
-            np.testing.assert_array_equal(state.reshape(-1), ostate, err_msg=f"compact={compact}")
+            np.testing.assert_array_equal(state.reshape(-1), ostate, err_msg=f"ring form {form}")
     return counts
 
 
@@ -181,7 +186,7 @@ def test_compact_ring_random_matches_oracle(oracle, case):
     g = oracle.OracleGraph.from_contacts(nn, a, b, s, e)
     oc, ostate, _ = g.simulate(_to_oracle_params(oracle, p, threads=8), want_state=True)
     with Simulation.from_contacts(nn, a, b, s, e, p) as sim:
-        assert sim.info()["ring_compact"] == (1 if sim.info()["ring_depth"] == 1 else 0)
+        assert sim.info()["ring_form"] in (replay_amd.RING_COMPACT, replay_amd.RING_OFFSETS)
         for v in (replay_amd.V_WORD, replay_amd.V_NODE4, replay_amd.V_NODE16, replay_amd.V_SLICED):
             sim.set_variant(v)
             sim.reset()
@@ -218,14 +223,13 @@ def test_ring_beyond_device_memory_runs_compact():
     args = (n, a.astype(np.int32), b.astype(np.int32), s.astype(np.int64), e.astype(np.int64))
     with Simulation.from_contacts(*args, p) as sim:
         info = sim.info()
-        print(f"T={T} ring_compact={info['ring_compact']} ring_bytes={info['ring_bytes']}",
-              flush=True)
-        assert info["ring_compact"] == 1 and info["ring_bytes"] < 16e9
+        print(f"T={T} ring_form={info['ring_form']} ring_bytes={info['ring_bytes']}", flush=True)
+        assert info["ring_form"] == replay_amd.RING_COMPACT and info["ring_bytes"] < 16e9
         sim.run()
         counts = sim.counts()
     from dataclasses import replace
     with Simulation.from_contacts(*args, replace(p, M=64)) as small:
-        assert small.info()["ring_compact"] == 0
+        assert small.info()["ring_form"] != replay_amd.RING_COMPACT
         small.run()
         c64 = small.counts()
     np.testing.assert_array_equal(counts[:, :64], c64)

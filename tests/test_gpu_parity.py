@@ -199,7 +199,8 @@ def test_random_sweep_every_variant(oracle, case):
     e = int(rng.choice([100, 600, 2500, 6000]))
     M = int(rng.choice([1, 33, 64, 130, 256, 1024]))
     step, D = SWEEP_WINDOWS[int(rng.integers(0, len(SWEEP_WINDOWS)))]
-    flags = int(rng.choice([0, replay_amd.F_WIDE_ENTRIES]))
+    flags = int(rng.choice([0, replay_amd.F_WIDE_ENTRIES])) | \
+        int(rng.choice([0, replay_amd.F_NO_HISTORY]))
     long_frac = float(rng.choice([0.0, 0.2, 0.6]))
     print(f"case {case}: seed={seed} n={n} e={e} M={M} step={step} D={D} flags={flags}", flush=True)
     nn, a, b, s, en = _random_case(seed, n=n, e=e, long_frac=long_frac)
