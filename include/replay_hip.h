@@ -132,7 +132,8 @@ typedef struct replay_info {
 #define REPLAY_RING_OFFSETS 0 /* slots back to the longest timer offset; each infection
                                  recorded per replica as it is drawn */
 #define REPLAY_RING_COMPACT 1 /* one slot plus a flag bit per word and iteration */
-#define REPLAY_RING_HISTORY 2 /* every iteration; per-replica records derived when read */
+#define REPLAY_RING_HISTORY 2 /* every iteration; per-replica records derived when read
+                                 (runs of up to 1024 iterations that fit max_ring_bytes) */
 
 /* replay_info.variant */
 #define REPLAY_V_WORD 0       /* one pass over a node's contacts per 64-replica word */

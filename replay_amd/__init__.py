@@ -76,6 +76,7 @@ F_AUTOTUNE = 2
 F_WIDE_ENTRIES = 4
 F_NO_HISTORY = 8
 RING_OFFSETS, RING_COMPACT, RING_HISTORY = 0, 1, 2
+HISTORY_MAX_ITERATIONS = 1024  # longest run the library keeps in the history form
 V_WORD, V_NODE4, V_NODE16, V_SLICED = 0, 4, 16, 1
 
 

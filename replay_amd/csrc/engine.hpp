@@ -63,9 +63,11 @@ struct StepArgs {
   const float *uniforms;      // injected stream for this slot, [M][N], or null
   const NetDev *nets;
   const int32_t *net_of;      // node -> network
+  const int32_t *node_word;   // node -> its first word (wsh + i * W of its network)
   const uint32_t *rank_limit; // [n_net] of this window, or null (no cap bites)
   const float *p_tab;         // [n_net][tab_max+1]: P of c whole windows, or null
-  const int32_t *upd_net;     // network of each update_kernel block (-1: none)
+  const int4 *upd;            // update_kernel block -> {network (-1: none), first word,
+                              // end of the network's words, 0}
   uint64_t *F;                // compact ring: [depthF][update waves] flag words, or null
   int32_t depthF;             // slots of F
   int32_t record;             // 1: each infection is written to entry[] and counted in
@@ -133,7 +135,7 @@ struct replay_ctx {
   bool history = false;
   int32_t tallied = 0;
   int32_t rebuilt = 0;  // iterations whose infections entry[] reflects (history ring)
-  int32_t *d_tally = nullptr;  // (network, word) pairs of the tally grid, as net << 16 | q
+  int32_t *d_tally = nullptr;  // tally blocks: network << 16 | first word of 4
   // merged plan and slices
   std::vector<int64_t> iter_time;
   std::vector<int32_t> iter_window;
@@ -173,7 +175,8 @@ struct replay_ctx {
   rp::Segs *d_segs = nullptr;
   rp::NetDev *d_nets = nullptr;
   int32_t *d_net_of = nullptr;
-  int32_t *d_upd_net = nullptr;   // network of each update_kernel block
+  int4 *d_upd = nullptr;          // update_kernel blocks: {network, first word, end, 0}
+  int32_t *d_node_word = nullptr; // first word of each node
   int32_t *d_col_net = nullptr;   // network of each replica column
   uint32_t *d_rank_limit = nullptr;  // [n_windows][n_net] when a cap bites
   float *d_p_tab = nullptr;          // [n_net][tab_max+1]

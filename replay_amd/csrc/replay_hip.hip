@@ -553,12 +553,17 @@ void expo_kernel(StepArgs a) {
   // two chunks of contact entries.
   const int e0 = a.rowptr[i], e1 = a.rowptr[i + 1];
   const int net = a.net_of[i];
+  // the node's first word: i * W when every network has the same replica
+  // count (a.M != 0: no table read before the state load), else its entry
+  // in the word table
+  const int64_t wi = a.M != 0 ? static_cast<int64_t>(i) * a.W : a.node_word[i];
+  // lanes past the node's words read the next node's (masked below), never
+  // past the last word
+  const uint64_t s_first = a.S[min(wi + min(lane, WT - 1), a.NW - 1)];
   const NetDev nd = a.nets[net];
-  const int W = nd.W;
-  const int64_t wi = nd.wsh + static_cast<int64_t>(i) * W;  // the node's first word
+  const int W = a.M != 0 ? a.W : nd.W;
   uint64_t *X_k = a.X + static_cast<size_t>(a.k % a.depth) * a.NW + wi;
   const uint64_t *S = a.S + wi;
-  const uint64_t s_first = S[min(lane, min(WT, W) - 1)];
   if (e1 - e0 > a.long_row) return;  // long_row_kernel takes it
   if (!kInjected && e1 <= e0) {
     // no contact in the epoch: no exposure, so no infection (a draw needs
@@ -964,7 +969,7 @@ __device__ __forceinline__ uint64_t infected_at(const int32_t *entry, int64_t w,
 // Phase 2 of an iteration: one thread per word applies the timer flips for
 // iteration k+1 and advances the running prefix of the infection counts,
 // P[k] = P[k-1] + inf_count[k-1].  Each block's words belong to one network
-// (a.upd_net names it; a network's words start at its own block), so the
+// (a.upd names it; a network's words start at its own block), so the
 // flip offsets are the block's.
 //
 // The flip at offset n >= 2 needs X(k+1-n), the replicas infected n-1
@@ -975,12 +980,14 @@ __device__ __forceinline__ uint64_t infected_at(const int32_t *entry, int64_t w,
 // replicas -- a replica is infected again only once susceptible, and that
 // comes at offset n_S, the last of the three.
 __global__ __launch_bounds__(kBlock) void update_kernel(StepArgs a) {
-  const int net = a.upd_net[blockIdx.x];
+  const int4 ub = a.upd[blockIdx.x];
+  const int net = ub.x;
   if (net >= 0) {
+    // the word loads need only the block's entry; the flip offsets come
+    // from the network's constants, loaded alongside
+    const int64_t w = static_cast<int64_t>(ub.y) + threadIdx.x;
+    const bool valid = w < ub.z;
     const NetDev nd = a.nets[net];
-    const int64_t j = static_cast<int64_t>(blockIdx.x - nd.blk0) * kBlock + threadIdx.x;
-    const int64_t w = nd.woff + j;
-    const bool valid = j < nd.nwords;
     const uint64_t x = valid ? a.X[static_cast<size_t>(a.k % a.depth) * a.NW + w] : 0ull;
     uint64_t inf = valid ? a.I_cur[w] : 0ull;
     uint64_t sus = valid ? a.S[w] & ~x : 0ull;
@@ -1035,42 +1042,60 @@ __global__ __launch_bounds__(kBlock) void update_kernel(StepArgs a) {
         a.P[static_cast<size_t>(a.k - 1) * G + g] + a.inf_count[static_cast<size_t>(a.k - 1) * G + g];
 }
 
-// History ring: infections of iterations j0 + blockIdx.y per replica.  A
-// block takes one (network, word q) pair (tally[blockIdx.x] = net << 16 | q);
-// its 4 wavefronts walk the network's nodes 64 at a time (lane = node), and
-// the column count of the 64 gathered X words gives each lane the number of
-// those nodes whose replica 64 q + lane was infected.  The wavefronts' sums
-// meet in LDS and lane r writes inf_count of replica column moff + 64 q + r:
-// every column of the iteration is written, no atomics.
-__global__ __launch_bounds__(kBlock) void tally_kernel(StepArgs a, const int32_t *tally,
-                                                       int32_t j0) {
-  __shared__ uint32_t part[kWavesPerBlock][kWave];
+// History ring: infections of iteration j0 + blockIdx.y per replica.  A
+// block takes one network's words q0 .. q0+3 (tally[blockIdx.x] = net << 16
+// | q0) and its 8 wavefronts walk the network's nodes in tiles of 64: a
+// wavefront loads a tile's 64 x 4 words with consecutive lanes on
+// consecutive words, turns it in its own LDS rows so that lane l holds node
+// l's word q, and the column count of that row gives each lane the number of
+// the tile's nodes whose replica 64 q + lane was infected.  The wavefronts'
+// sums meet in LDS and replica column moff + 64 q + r is written once: no
+// atomics, every column of the iteration written.
+constexpr int kTallyWaves = 8;
+constexpr int32_t kHistoryMaxIterations = 1024;  // longest run kept in the history form
+__global__ __launch_bounds__(kTallyWaves * kWave) void tally_kernel(StepArgs a,
+                                                                    const int32_t *tally,
+                                                                    int32_t j0) {
+  constexpr int kRow = kWave + 1;  // padded LDS row: the transposing writes spread over banks
+  __shared__ uint64_t tile[kTallyWaves][4][kRow];
+  __shared__ uint32_t part[kTallyWaves][4][kWave];
   const int lane = threadIdx.x & (kWave - 1);
   const int wv = threadIdx.x / kWave;
   const int32_t t = tally[blockIdx.x];
-  const int net = t >> 16, q = t & 0xFFFF;
+  const int net = t >> 16, q0 = t & 0xFFFF;
   const NetDev nd = a.nets[net];
+  const int nq = min(4, nd.W - q0);
   const int32_t j = j0 + static_cast<int32_t>(blockIdx.y);
-  const uint64_t *X = a.X + static_cast<size_t>(j % a.depth) * a.NW;
+  const uint64_t *X = a.X + static_cast<size_t>(j % a.depth) * a.NW + nd.woff;
   const int32_t n_nodes = static_cast<int32_t>(nd.nwords / nd.W);
-  uint32_t acc = 0;
-  for (int32_t c0 = wv * kWave; c0 < n_nodes; c0 += 4 * kBlock) {
-    // up to four 64-node chunks per pass, their loads in flight together
-    uint64_t x[4];
+  uint32_t acc[4] = {0u, 0u, 0u, 0u};
+  for (int32_t n0 = wv * kWave; n0 < n_nodes; n0 += kTallyWaves * kWave) {
 #pragma unroll
-    for (int u = 0; u < 4; ++u) {
-      const int32_t node = c0 + u * kBlock + lane;
-      x[u] = node < n_nodes ? X[nd.woff + static_cast<int64_t>(node) * nd.W + q] : 0ull;
+    for (int u = 0; u < 4; ++u) {  // tile word u * 64 + lane: node / 4, word % 4
+      const int idx = u * kWave + lane;
+      const int32_t node = n0 + (idx >> 2);
+      const int qq = idx & 3;
+      tile[wv][qq][idx >> 2] =
+          (node < n_nodes && qq < nq) ? X[static_cast<int64_t>(node) * nd.W + q0 + qq] : 0ull;
     }
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
 #pragma unroll
-    for (int u = 0; u < 4; ++u)
-      if (__ballot(x[u] != 0ull) != 0ull) acc += column_count(x[u], lane);
+    for (int qq = 0; qq < 4; ++qq) {
+      const uint64_t x = tile[wv][qq][lane];
+      if (__ballot(x != 0ull) != 0ull) acc[qq] += column_count(x, lane);
+    }
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
   }
-  part[wv][lane] = acc;
+#pragma unroll
+  for (int qq = 0; qq < 4; ++qq) part[wv][qq][lane] = acc[qq];
   __syncthreads();
-  if (wv == 0) {
-    const uint32_t sum = part[0][lane] + part[1][lane] + part[2][lane] + part[3][lane];
-    const int r = q * kWave + lane;
+  if (wv < nq) {  // wavefront qq sums word q0 + qq over the 8 wavefronts
+    uint32_t sum = 0;
+#pragma unroll
+    for (int v = 0; v < kTallyWaves; ++v) sum += part[v][wv][lane];
+    const int r = (q0 + wv) * kWave + lane;
     if (r < nd.M) a.inf_count[static_cast<size_t>(j) * a.G + nd.moff + r] = static_cast<int32_t>(sum);
   }
 }
@@ -1291,7 +1316,8 @@ void release(replay_ctx *ctx) {
                   ctx->d_X,       ctx->d_Xinit,  ctx->d_entry,  ctx->d_inf,         ctx->d_P,
                   ctx->d_init_count, ctx->d_counts, ctx->d_class_init, ctx->d_segs, ctx->d_nets,
                   ctx->d_net_of,  ctx->d_rank_limit, ctx->d_uniforms, ctx->d_p_tab,
-                  ctx->d_long_rows, ctx->d_upd_net, ctx->d_col_net, ctx->d_F, ctx->d_tally};
+                  ctx->d_long_rows, ctx->d_upd, ctx->d_col_net, ctx->d_F, ctx->d_tally,
+                  ctx->d_node_word};
   for (void *b : bufs) (void)hipFree(b);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
   ctx->stream = nullptr;
@@ -1381,9 +1407,11 @@ StepArgs rp::step_args(replay_ctx *ctx, int32_t k) {
                    : nullptr;
   a.nets = ctx->d_nets;
   a.net_of = ctx->d_net_of;
+  a.node_word = ctx->d_node_word;
   a.rank_limit = ctx->any_cut ? ctx->d_rank_limit + static_cast<size_t>(win) * ctx->n_net : nullptr;
   a.p_tab = ctx->d_p_tab;
-  a.upd_net = ctx->d_upd_net;
+  a.upd = ctx->d_upd;
+  a.node_word = ctx->d_node_word;
   a.F = ctx->d_F;
   a.depthF = ctx->depthF;
   a.record = ctx->history ? 0 : 1;
@@ -1608,9 +1636,10 @@ int enqueue_counts(replay_ctx *ctx) {
       StepArgs a = step_args(ctx, 0);
       const dim3 grid(static_cast<unsigned>(std::accumulate(
                           ctx->nets.begin(), ctx->nets.end(), 0,
-                          [](int acc, const NetHost &nh) { return acc + nh.W; })),
+                          [](int acc, const NetHost &nh) { return acc + (nh.W + 3) / 4; })),
                       static_cast<unsigned>(j1 - t0));
-      hipLaunchKernelGGL(tally_kernel, grid, dim3(kBlock), 0, ctx->stream, a, ctx->d_tally, t0);
+      hipLaunchKernelGGL(tally_kernel, grid, dim3(kTallyWaves * kWave), 0, ctx->stream, a,
+                         ctx->d_tally, t0);
       hipLaunchKernelGGL(prefix_scan_kernel, dim3(nb), dim3(tb), 0, ctx->stream, ctx->d_P,
                          ctx->d_inf, G, t0, j1);
       ctx->tallied = j1;
@@ -1738,16 +1767,23 @@ int finish_open(replay_ctx *ctx) {
   ctx->G = static_cast<int32_t>(moff);
   // update_kernel's blocks: each network's words start at a block of their own
   std::vector<int32_t> blk0(static_cast<size_t>(ctx->n_net));
-  std::vector<int32_t> upd_net;
+  std::vector<int4> upd;
   for (int32_t n = 0; n < ctx->n_net; ++n) {
     const NetHost &nh = ctx->nets[n];
-    blk0[n] = static_cast<int32_t>(upd_net.size());
-    const int64_t nb = (static_cast<int64_t>(nh.net.n_nodes) * nh.W + kBlock - 1) / kBlock;
-    upd_net.insert(upd_net.end(), static_cast<size_t>(nb), n);
+    blk0[n] = static_cast<int32_t>(upd.size());
+    const int64_t end = nh.woff + static_cast<int64_t>(nh.net.n_nodes) * nh.W;
+    for (int64_t w = nh.woff; w < end; w += kBlock)
+      upd.push_back(make_int4(n, static_cast<int32_t>(w), static_cast<int32_t>(end), 0));
   }
-  ctx->upd_blocks = static_cast<int32_t>(upd_net.size());
-  upd_net.resize(std::max<size_t>(upd_net.size(), static_cast<size_t>((ctx->G + kBlock - 1) / kBlock)),
-                 -1);
+  ctx->upd_blocks = static_cast<int32_t>(upd.size());
+  upd.resize(std::max<size_t>(upd.size(), static_cast<size_t>((ctx->G + kBlock - 1) / kBlock)),
+             make_int4(-1, 0, 0, 0));
+  std::vector<int32_t> node_word(static_cast<size_t>(ctx->N));
+  for (const NetHost &nh : ctx->nets)
+    for (int32_t r = 0; r < nh.net.n_nodes; ++r)
+      node_word[static_cast<size_t>(nh.node_off) + r] =
+          static_cast<int32_t>(nh.woff + static_cast<int64_t>(r) * nh.W);
+
   std::vector<int32_t> col_net(static_cast<size_t>(ctx->G));
   for (int32_t n = 0; n < ctx->n_net; ++n)
     std::fill(col_net.begin() + ctx->nets[n].moff, col_net.begin() + ctx->nets[n].moff + ctx->nets[n].M,
@@ -1899,12 +1935,15 @@ int finish_open(replay_ctx *ctx) {
   const int64_t ring_budget = p.max_ring_bytes > 0 ? p.max_ring_bytes : (int64_t(16) << 30);
   const size_t upd_waves = static_cast<size_t>(step_grid(ctx).x) * kWavesPerBlock;
   std::vector<int32_t> tally;
-  if (ctx->T >= 1 && !(p.flags & REPLAY_F_NO_HISTORY) &&
+  // The tally's running prefix walks the iterations in sequence, which on
+  // long runs (thousands of iterations) costs more than the per-infection
+  // records it saves: those keep the offsets ring.
+  if (ctx->T >= 1 && ctx->T <= kHistoryMaxIterations && !(p.flags & REPLAY_F_NO_HISTORY) &&
       static_cast<int64_t>(NW) * 8 * (ctx->T + 1) <= ring_budget) {
     ctx->history = true;
     ctx->depth = ctx->T + 1;
     for (int32_t n = 0; n < ctx->n_net; ++n)
-      for (int32_t q = 0; q < ctx->nets[n].W; ++q) tally.push_back(n << 16 | q);
+      for (int32_t q = 0; q < ctx->nets[n].W; q += 4) tally.push_back(n << 16 | q);
     if (ctx->n_net >= (1 << 15) || ctx->W >= (1 << 16))
       return fail(ctx, REPLAY_E_UNSUPPORTED, "too many networks or words for the history tally");
   } else if (static_cast<int64_t>(NW) * 8 * ctx->depth > ring_budget && ctx->depth > 1) {
@@ -1920,7 +1959,8 @@ int finish_open(replay_ctx *ctx) {
       (rc = upload(ctx, &ctx->d_segs, segs.data(), segs.size())) ||
       (rc = upload(ctx, &ctx->d_class_init, class_init.data(), class_init.size())) ||
       (rc = upload(ctx, &ctx->d_net_of, net_of.data(), net_of.size())) ||
-      (rc = upload(ctx, &ctx->d_upd_net, upd_net.data(), upd_net.size())) ||
+      (rc = upload(ctx, &ctx->d_upd, upd.data(), upd.size())) ||
+      (rc = upload(ctx, &ctx->d_node_word, node_word.data(), node_word.size())) ||
       (rc = upload(ctx, &ctx->d_col_net, col_net.data(), col_net.size())) ||
       (rc = (ctx->any_cut ? upload(ctx, &ctx->d_rank_limit, limits.data(), limits.size())
                           : REPLAY_OK)) ||

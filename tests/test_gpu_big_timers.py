@@ -56,13 +56,16 @@ def first_at_least(seq: np.ndarray, upper: int, elapsed: int) -> int:
 
 def _run_case(oracle, n, a, b, s, e, p, want_states=True):
     """The oracle's counts and states, bit for bit, with every form of the
-    infection ring: every iteration kept (the default here), slots back to
-    the longest timer offset (REPLAY_F_NO_HISTORY), and the compact form
-    (forced by a one-byte ring budget)."""
+    infection ring: every iteration kept (the default for runs of up to
+    HISTORY_MAX_ITERATIONS), slots back to the longest timer offset
+    (REPLAY_F_NO_HISTORY, and the default for longer runs), and the compact
+    form (forced by a one-byte ring budget)."""
     from dataclasses import replace
     g = oracle.OracleGraph.from_contacts(n, a, b, s, e)
     oc, ostate, _ = g.simulate(_to_oracle_params(oracle, p, threads=8), want_state=want_states)
-    forms = ((dict(), replay_amd.RING_HISTORY),
+    default = (replay_amd.RING_HISTORY if p.iterations <= replay_amd.HISTORY_MAX_ITERATIONS
+               else replay_amd.RING_OFFSETS)
+    forms = ((dict(), default),
              (dict(flags=replay_amd.F_NO_HISTORY), replay_amd.RING_OFFSETS),
              (dict(max_ring_bytes=1), replay_amd.RING_COMPACT))
     for kw, form in forms:
