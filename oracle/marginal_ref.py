@@ -52,3 +52,41 @@ def simulate(A: torch.Tensor, beta, sigma, gamma, phi, p0, dt_h: float) -> torch
         s, e, io, r = s * (1.0 - lam), e * (1.0 - a) + s * lam, io * (1.0 - c) + e * a, r + io * c
         rows.append(torch.stack([s.sum(0), e.sum(0), io.sum(0), r.sum(0)], dim=1))
     return torch.stack(rows, dim=0)
+
+
+def sparse_slices(graph, params, iterations: int) -> list:
+    """The same slices as sparse (N, N) float64 COO tensors, one per
+    iteration, for graphs too large for dense_slices (duplicate CSR entries
+    are summed, as dense_slices sums them)."""
+    N = graph.n_nodes
+    out = []
+    for k in range(iterations):
+        rowptr, col, val, _ = graph.step_csr(params, k)
+        rows = np.repeat(np.arange(N, dtype=np.int64), np.diff(rowptr).astype(np.int64))
+        idx = torch.from_numpy(np.stack([rows, col.astype(np.int64)]))
+        A = torch.sparse_coo_tensor(idx, torch.from_numpy(val.astype(np.float64)), (N, N))
+        out.append(A.coalesce())
+    return out
+
+
+def simulate_sparse(slices: list, beta, sigma, gamma, phi, p0, dt_h: float) -> torch.Tensor:
+    """simulate() with the exposure product taken over sparse slices
+    (torch.sparse.mm, which carries autograd to the dense operand)."""
+    N = slices[0].shape[0] if slices else phi.shape[0]
+    B = beta.shape[0]
+    dt = beta.dtype
+    p0 = torch.as_tensor(p0, dtype=dt).expand(N)[:, None].expand(N, B)
+    s = 1.0 - p0
+    e = torch.zeros(N, B, dtype=dt)
+    io = p0.clone()
+    r = torch.zeros(N, B, dtype=dt)
+    a = -torch.expm1(-sigma * dt_h)
+    c = -torch.expm1(-gamma * dt_h)
+    kappa = (beta / 3600.0)[None, :] * phi[:, None]
+    rows = [torch.stack([s.sum(0), e.sum(0), io.sum(0), r.sum(0)], dim=1)]
+    for A in slices:
+        E = torch.sparse.mm(A.to(dt), io)
+        lam = -torch.expm1(-kappa * E)
+        s, e, io, r = s * (1.0 - lam), e * (1.0 - a) + s * lam, io * (1.0 - c) + e * a, r + io * c
+        rows.append(torch.stack([s.sum(0), e.sum(0), io.sum(0), r.sum(0)], dim=1))
+    return torch.stack(rows, dim=0)

@@ -233,4 +233,37 @@ int upload(replay_ctx *ctx, T **ptr, const T *src, size_t count) {
 // buffers of the current parity, the draw slot.
 StepArgs step_args(replay_ctx *ctx, int32_t k);
 
+typedef int v4i __attribute__((ext_vector_type(4)));
+
+// Contact entries e .. e+63 of a row that ends at e1 (e < e1), one per lane;
+// lanes past the row re-read its last entry and are masked by the reader.
+// Streamed once per iteration: kept out of the caches' way.
+__device__ __forceinline__ v4i load_chunk(const int4 *entries, int e, int e1, int lane) {
+  return __builtin_nontemporal_load(reinterpret_cast<const v4i *>(entries + min(e + lane, e1 - 1)));
+}
+
+typedef unsigned int v2u __attribute__((ext_vector_type(2)));
+
+// The same chunk from the 8-byte copy of the entries.
+__device__ __forceinline__ v2u load_chunk(const uint2 *entries8, int e, int e1, int lane) {
+  return __builtin_nontemporal_load(reinterpret_cast<const v2u *>(entries8 + min(e + lane, e1 - 1)));
+}
+
+// Fields of a packed entry: {col, ts, te, rank} from the 16-byte form; from
+// the 8-byte form {col | ts << 24, ts >> 8 | te << 12} (24-bit column,
+// 20-bit epoch-relative times), whose contexts have no rank cap.
+struct EntryFields {
+  int col, ts, te;
+  uint32_t rank;
+};
+__device__ __forceinline__ EntryFields fields(const v4i &en) {
+  return EntryFields{en.x, en.y, en.z, static_cast<uint32_t>(en.w)};
+}
+__device__ __forceinline__ EntryFields fields(const v2u &en) {
+  return EntryFields{static_cast<int>(en.x & 0xFFFFFFu),
+                     static_cast<int>((en.x >> 24) | ((en.y & 0xFFFu) << 8)),
+                     static_cast<int>(en.y >> 12), 0u};
+}
+
+
 }  // namespace rp

@@ -47,6 +47,7 @@ namespace {
 struct MfStep {
   const int32_t *rowptr;
   const int4 *entries;
+  const uint2 *entries8;  // the same entries in 8 bytes, or null
   const uint32_t *rank_limit;
   const int32_t *net_of;
   int32_t ws_rel, we_rel;
@@ -55,20 +56,56 @@ struct MfStep {
 
 MfStep mf_step(replay_ctx *ctx, int32_t k) {
   const StepArgs s = step_args(ctx, k);
-  return MfStep{s.rowptr, s.entries, s.rank_limit, s.net_of, s.ws_rel, s.we_rel, s.scale};
+  return MfStep{s.rowptr, s.entries, s.entries8, s.rank_limit, s.net_of, s.ws_rel, s.we_rel,
+                s.scale};
 }
 
-// sum_j A_k[row i, j] * v[j][b] in entry order (ascending column).
+constexpr int kRowBatch = 8;  // neighbour rows requested together
+
+// sum_j A_k[row i, j] * v[j][b] in entry order (ascending column).  The
+// wavefront loads 64 of the row's entries at a time (one per lane, from the
+// 8-byte copy where EnT is v2u), then takes the counting ones in order,
+// kRowBatch at a time: their neighbour rows (one coalesced load of the Bp
+// sets each) are requested together and then added in entry order, so the
+// sum is the sequential one, term for term.
+template <typename EnT>
 __device__ __forceinline__ float row_product(const MfStep &g, int i, const float *v, int Bp,
-                                             int b) {
+                                             int b, int lane) {
   const int e0 = g.rowptr[i], e1 = g.rowptr[i + 1];
   const uint32_t rlim = g.rank_limit ? g.rank_limit[g.net_of[i]] : UINT32_MAX;
+  const auto *src = [&] {
+    if constexpr (sizeof(EnT) == 8)
+      return g.entries8;
+    else
+      return g.entries;
+  }();
   float acc = 0.0f;
-  for (int e = e0; e < e1; ++e) {
-    const int4 en = g.entries[e];
-    const int ov = min(en.z, g.we_rel) - max(en.y, g.ws_rel);
-    if (ov > 0 && static_cast<uint32_t>(en.w) <= rlim)
-      acc += (static_cast<float>(ov) * g.scale) * v[static_cast<size_t>(en.x) * Bp + b];
+  for (int c0 = e0; c0 < e1; c0 += kWave) {
+    const EntryFields en = fields(load_chunk(src, c0, e1, lane));
+    const int ov = min(en.te, g.we_rel) - max(en.ts, g.ws_rel);
+    const bool on = c0 + lane < e1 && ov > 0 && en.rank <= rlim;
+    const float w = on ? static_cast<float>(ov) * g.scale : 0.0f;
+    const int col = on ? en.col : 0;
+    for (uint64_t act = __ballot(on); act != 0ull;) {
+      float wj[kRowBatch], vj[kRowBatch];
+      int n = 0;
+#pragma unroll
+      for (int u = 0; u < kRowBatch; ++u) {
+        wj[u] = 0.0f;
+        vj[u] = 0.0f;
+        if (act != 0ull) {
+          const int j = __builtin_ctzll(act);
+          act &= act - 1;
+          const int cj = __builtin_amdgcn_readlane(col, j);
+          wj[u] = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(w), j));
+          vj[u] = v[static_cast<size_t>(cj) * Bp + b];
+          n = u + 1;
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < kRowBatch; ++u)
+        if (u < n) acc += wj[u] * vj[u];
+    }
   }
   return acc;
 }
@@ -94,10 +131,16 @@ __global__ __launch_bounds__(kBlock) void mf_prep_kernel(const float *beta, cons
   if (t < N) phi[t] = phi_in[t];
 }
 
+// Blocks of the forward and initial-state kernels: 16 wavefronts, so that
+// many nodes are in flight while the per-block partials of the curves stay
+// as few as with 4-wavefront blocks over a quarter of the wavefronts.
+constexpr int kMfWaves = 16;
+constexpr int kMfBlock = kMfWaves * kWave;
+
 // Adds the four compartment sums of the block's wavefronts in wavefront
 // order and writes them as the block's partial part[block][b][4].
 __device__ __forceinline__ void block_partial(const float (&sum)[4], int b, int Bp, float *part) {
-  __shared__ float red[kWavesPerBlock][kWave][4];
+  __shared__ float red[kMfWaves][kWave][4];
   const int lane = threadIdx.x & (kWave - 1);
   const int wv = threadIdx.x / kWave;
 #pragma unroll
@@ -108,7 +151,7 @@ __device__ __forceinline__ void block_partial(const float (&sum)[4], int b, int 
     for (int c = 0; c < 4; ++c) {
       float acc = red[0][lane][c];
 #pragma unroll
-      for (int w = 1; w < kWavesPerBlock; ++w) acc += red[w][lane][c];
+      for (int w = 1; w < kMfWaves; ++w) acc += red[w][lane][c];
       part[(static_cast<size_t>(blockIdx.x) * Bp + b) * 4 + c] = acc;
     }
   }
@@ -116,12 +159,12 @@ __device__ __forceinline__ void block_partial(const float (&sum)[4], int b, int 
 }
 
 // Initial state and its curve partials.
-__global__ __launch_bounds__(kBlock) void mf_init_kernel(float *st, const NetDev *nets,
-                                                         const int32_t *net_of, float *part,
-                                                         int N, int Bp) {
+__global__ __launch_bounds__(kMfBlock) void mf_init_kernel(float *st, const NetDev *nets,
+                                                           const int32_t *net_of, float *part,
+                                                           int N, int Bp) {
   const int lane = threadIdx.x & (kWave - 1);
-  const int wave = blockIdx.x * kWavesPerBlock + static_cast<int>(threadIdx.x / kWave);
-  const int n_waves = gridDim.x * kWavesPerBlock;
+  const int wave = blockIdx.x * kMfWaves + static_cast<int>(threadIdx.x / kWave);
+  const int n_waves = gridDim.x * kMfWaves;
   const size_t NB = static_cast<size_t>(N) * Bp;
   for (int cb = 0; cb < Bp; cb += kWave) {
     const int b = cb + lane;
@@ -141,19 +184,20 @@ __global__ __launch_bounds__(kBlock) void mf_init_kernel(float *st, const NetDev
 }
 
 // One forward step k -> k+1; optionally records the exposure of k.
-__global__ __launch_bounds__(kBlock) void mf_forward_kernel(MfStep g, MfParams p, const float *cur,
-                                                            float *next, float *E_out,
-                                                            float *part) {
+template <typename EnT>
+__global__ __launch_bounds__(kMfBlock) void mf_forward_kernel(MfStep g, MfParams p,
+                                                              const float *cur, float *next,
+                                                              float *E_out, float *part) {
   const int lane = threadIdx.x & (kWave - 1);
-  const int wave = blockIdx.x * kWavesPerBlock + static_cast<int>(threadIdx.x / kWave);
-  const int n_waves = gridDim.x * kWavesPerBlock;
+  const int wave = blockIdx.x * kMfWaves + static_cast<int>(threadIdx.x / kWave);
+  const int n_waves = gridDim.x * kMfWaves;
   const size_t NB = static_cast<size_t>(p.N) * p.Bp;
   for (int cb = 0; cb < p.Bp; cb += kWave) {
     const int b = cb + lane;
     const float kb = p.kb[b], a = p.ac[b], c = p.ac[p.Bp + b];
     float sum[4] = {0.0f, 0.0f, 0.0f, 0.0f};
     for (int i = wave; i < p.N; i += n_waves) {
-      const float E = row_product(g, i, cur + 2 * NB, p.Bp, b);
+      const float E = row_product<EnT>(g, i, cur + 2 * NB, p.Bp, b, lane);
       const size_t o = static_cast<size_t>(i) * p.Bp + b;
       const float s = cur[o], e = cur[NB + o], io = cur[2 * NB + o], r = cur[3 * NB + o];
       const float lam = -expm1f(-(kb * p.phi[i]) * E);
@@ -258,16 +302,17 @@ __global__ __launch_bounds__(kBlock) void mf_adjoint_kernel(MfParams p, const fl
 }
 
 // Adjoint phase 2 of step k: adj_out[iota][j] += sum_i A_k[j,i] G_i.
-__global__ __launch_bounds__(kBlock) void mf_spmm_adj_kernel(MfStep g, MfParams p, const float *G,
-                                                             float *adj_out) {
+template <typename EnT>
+__global__ __launch_bounds__(kMfBlock) void mf_spmm_adj_kernel(MfStep g, MfParams p,
+                                                               const float *G, float *adj_out) {
   const int lane = threadIdx.x & (kWave - 1);
-  const int wave = blockIdx.x * kWavesPerBlock + static_cast<int>(threadIdx.x / kWave);
-  const int n_waves = gridDim.x * kWavesPerBlock;
+  const int wave = blockIdx.x * kMfWaves + static_cast<int>(threadIdx.x / kWave);
+  const int n_waves = gridDim.x * kMfWaves;
   const size_t NB = static_cast<size_t>(p.N) * p.Bp;
   for (int cb = 0; cb < p.Bp; cb += kWave) {
     const int b = cb + lane;
     for (int j = wave; j < p.N; j += n_waves) {
-      const float t = row_product(g, j, G, p.Bp, b);
+      const float t = row_product<EnT>(g, j, G, p.Bp, b, lane);
       adj_out[2 * NB + static_cast<size_t>(j) * p.Bp + b] += t;
     }
   }
@@ -304,6 +349,7 @@ __global__ __launch_bounds__(kBlock) void mf_grad_kernel(const float *pg, int n_
 struct replay_marginal {
   replay_ctx *ctx = nullptr;
   int32_t B = 0, Bp = 0, T = 0, every = 0, N = 0, n_waves = 0, n_ckpt = 0;
+  int32_t f_blocks = 0;  // blocks of the forward kernel (and curve partials per step)
   float dt_h = 0.0f;
   bool have_forward = false;
   float *d_st[2] = {nullptr, nullptr};  // [4][N][Bp]
@@ -335,6 +381,28 @@ MfParams mf_params(const replay_marginal *m) {
 // which is also what a torch process passes for its default stream.
 hipStream_t mf_stream(void *stream) { return static_cast<hipStream_t>(stream); }
 
+// The forward step and the transposed product, streaming the 8-byte
+// entries where the context holds them.
+void launch_forward(const MfStep &g, const MfParams &p, const float *cur, float *next, float *E_out,
+                    float *part, unsigned blocks, hipStream_t st) {
+  if (g.entries8)
+    hipLaunchKernelGGL(mf_forward_kernel<v2u>, dim3(blocks), dim3(kMfBlock), 0, st, g, p, cur,
+                       next, E_out, part);
+  else
+    hipLaunchKernelGGL(mf_forward_kernel<v4i>, dim3(blocks), dim3(kMfBlock), 0, st, g, p, cur,
+                       next, E_out, part);
+}
+
+void launch_spmm_adj(const MfStep &g, const MfParams &p, const float *G, float *adj_out,
+                     unsigned blocks, hipStream_t st) {
+  if (g.entries8)
+    hipLaunchKernelGGL(mf_spmm_adj_kernel<v2u>, dim3(blocks), dim3(kMfBlock), 0, st, g, p, G,
+                       adj_out);
+  else
+    hipLaunchKernelGGL(mf_spmm_adj_kernel<v4i>, dim3(blocks), dim3(kMfBlock), 0, st, g, p, G,
+                       adj_out);
+}
+
 void zero(float *p, size_t n, hipStream_t st) {
   const unsigned blocks = static_cast<unsigned>(std::min<size_t>((n + 255) / 256, 4096));
   hipLaunchKernelGGL(mf_zero_kernel, dim3(std::max(1u, blocks)), dim3(256), 0, st, p, n);
@@ -361,6 +429,7 @@ int32_t replay_marginal_create(replay_ctx *ctx, int32_t n_sets, int32_t checkpoi
   m->dt_h = ctx->p.step_size / 3600.0f;
   m->n_waves = std::max(1, std::min(ctx->N, 4096)) / kWavesPerBlock * kWavesPerBlock;
   if (m->n_waves == 0) m->n_waves = kWavesPerBlock;
+  m->f_blocks = std::max(1, std::min(ctx->N, 16384) / kMfWaves);
   const size_t plane = static_cast<size_t>(m->N) * m->Bp;
   int rc = REPLAY_OK;
   if ((rc = dev_alloc(ctx, &m->d_st[0], 4 * plane)) || (rc = dev_alloc(ctx, &m->d_st[1], 4 * plane)) ||
@@ -369,7 +438,7 @@ int32_t replay_marginal_create(replay_ctx *ctx, int32_t n_sets, int32_t checkpoi
       (rc = dev_alloc(ctx, &m->d_segE, plane * m->every)) ||
       (rc = dev_alloc(ctx, &m->d_adj[0], 4 * plane)) || (rc = dev_alloc(ctx, &m->d_adj[1], 4 * plane)) ||
       (rc = dev_alloc(ctx, &m->d_G, plane)) ||
-      (rc = dev_alloc(ctx, &m->d_part, static_cast<size_t>(m->n_waves) * m->Bp * 4)) ||
+      (rc = dev_alloc(ctx, &m->d_part, static_cast<size_t>(m->f_blocks) * m->Bp * 4)) ||
       (rc = dev_alloc(ctx, &m->d_pg, static_cast<size_t>(m->n_waves) * m->Bp * 3)) ||
       (rc = dev_alloc(ctx, &m->d_phig, static_cast<size_t>(m->N))) ||
       (rc = dev_alloc(ctx, &m->d_kb, static_cast<size_t>(m->Bp))) ||
@@ -391,14 +460,14 @@ int32_t replay_marginal_forward(replay_marginal *m, const float *beta, const flo
   HIP_OR_FAIL(ctx, hipSetDevice(ctx->p.device));
   const hipStream_t st = mf_stream(stream);
   const size_t plane = static_cast<size_t>(m->N) * m->Bp;
-  const unsigned blocks = static_cast<unsigned>(m->n_waves / kWavesPerBlock);
+  const unsigned blocks = static_cast<unsigned>(m->f_blocks);
   const unsigned red_blocks = static_cast<unsigned>(m->B * 4);
-  const int n_parts = m->n_waves / kWavesPerBlock;
+  const int n_parts = m->f_blocks;
   const int prep_n = std::max(m->Bp, m->N);
   hipLaunchKernelGGL(mf_prep_kernel, dim3((prep_n + kBlock - 1) / kBlock), dim3(kBlock), 0, st,
                      beta, sigma, gamma, phi, m->d_kb, m->d_ac, m->d_phi, m->B, m->Bp, m->N,
                      m->dt_h);
-  hipLaunchKernelGGL(mf_init_kernel, dim3(blocks), dim3(kBlock), 0, st, m->d_st[0], ctx->d_nets,
+  hipLaunchKernelGGL(mf_init_kernel, dim3(blocks), dim3(kMfBlock), 0, st, m->d_st[0], ctx->d_nets,
                      ctx->d_net_of, m->d_part, m->N, m->Bp);
   hipLaunchKernelGGL(mf_reduce_kernel, dim3(red_blocks), dim3(kBlock), 0, st, m->d_part, n_parts,
                      m->Bp, curves);
@@ -409,8 +478,8 @@ int32_t replay_marginal_forward(replay_marginal *m, const float *beta, const flo
       HIP_OR_FAIL(ctx, hipMemcpyAsync(m->d_ckpt + static_cast<size_t>(k / m->every) * 4 * plane,
                                       m->d_st[cur], 4 * plane * sizeof(float),
                                       hipMemcpyDeviceToDevice, st));
-    hipLaunchKernelGGL(mf_forward_kernel, dim3(blocks), dim3(kBlock), 0, st, mf_step(ctx, k), p,
-                       m->d_st[cur], m->d_st[cur ^ 1], static_cast<float *>(nullptr), m->d_part);
+    launch_forward(mf_step(ctx, k), p, m->d_st[cur], m->d_st[cur ^ 1], nullptr, m->d_part, blocks,
+                   st);
     hipLaunchKernelGGL(mf_reduce_kernel, dim3(red_blocks), dim3(kBlock), 0, st, m->d_part, n_parts,
                        m->Bp, curves + static_cast<size_t>(k + 1) * m->B * 4);
     cur ^= 1;
@@ -443,9 +512,9 @@ int32_t replay_marginal_backward(replay_marginal *m, const float *grad_curves, f
                                     4 * plane * sizeof(float), hipMemcpyDeviceToDevice, st));
     for (int32_t k = k0; k < k1; ++k) {
       const size_t j = static_cast<size_t>(k - k0);
-      hipLaunchKernelGGL(mf_forward_kernel, dim3(blocks), dim3(kBlock), 0, st, mf_step(ctx, k), p,
-                         m->d_seg + j * 4 * plane, m->d_seg + (j + 1) * 4 * plane,
-                         m->d_segE + j * plane, m->d_part);
+      launch_forward(mf_step(ctx, k), p, m->d_seg + j * 4 * plane,
+                     m->d_seg + (j + 1) * 4 * plane, m->d_segE + j * plane, m->d_part,
+                     static_cast<unsigned>(m->f_blocks), st);
     }
     for (int32_t k = k1 - 1; k >= k0; --k) {
       const size_t j = static_cast<size_t>(k - k0);
@@ -453,8 +522,8 @@ int32_t replay_marginal_backward(replay_marginal *m, const float *grad_curves, f
                          m->d_seg + j * 4 * plane, m->d_segE + j * plane, m->d_adj[a],
                          grad_curves + static_cast<size_t>(k + 1) * m->B * 4, m->B,
                          m->d_adj[a ^ 1], m->d_G, m->d_pg, m->d_phig);
-      hipLaunchKernelGGL(mf_spmm_adj_kernel, dim3(blocks), dim3(kBlock), 0, st, mf_step(ctx, k), p,
-                         m->d_G, m->d_adj[a ^ 1]);
+      launch_spmm_adj(mf_step(ctx, k), p, m->d_G, m->d_adj[a ^ 1],
+                      static_cast<unsigned>(m->f_blocks), st);
       a ^= 1;
     }
   }

@@ -219,6 +219,41 @@ def test_config4_full_size_directional_derivatives():
         one.close()
 
 
+def test_config4_full_size_gradients_match_fp64(oracle):
+    """The 20k-node block network of the marginal workload over its first 64
+    iterations (20-minute steps), 64 parameter sets: curves within 1e-5 of
+    the node count and the adjoint's gradients (beta, sigma, gamma and the
+    20,000 susceptibilities) within 1e-4 relative of FP64 autograd over the
+    oracle's sparse slices."""
+    from dataclasses import replace
+    from replay_amd import workloads
+    w = workloads.config4_marginal()
+    g = w.graph
+    T = 64
+    p = replace(g.params, iterations=T)
+    og = oracle.OracleGraph.from_contacts(g.n_nodes, g.src, g.tgt, g.start, g.end)
+    slices = marginal_ref.sparse_slices(og, _to_oracle_params(oracle, p), T)
+    N, B = g.n_nodes, int(w.beta.size)
+    rng = np.random.default_rng(11)
+    params = [torch.tensor(x, dtype=torch.float64) for x in
+              (w.beta, w.sigma, w.gamma, rng.uniform(0.5, 1.5, N))]
+    W = torch.tensor(rng.normal(size=(T + 1, B, 4)))
+    xs = [t.clone().requires_grad_(True) for t in params]
+    C_ref = marginal_ref.simulate_sparse(slices, *xs, p.initial_infected, p.step_size / 3600.0)
+    g_ref = torch.autograd.grad((C_ref * W).sum(), xs)
+    print(f"fp64 reference done: N={N} B={B} T={T} nnz(k=0)={slices[0]._nnz()}", flush=True)
+    with Simulation.from_contacts(g.n_nodes, g.src, g.tgt, g.start, g.end, p) as sim:
+        eng = MarginalEngine(sim, B, checkpoint_every=32)
+        C = eng.forward(*[x.to(torch.float32).to(DEV) for x in params])
+        grads = eng.backward(W.to(torch.float32).to(DEV))
+        torch.cuda.synchronize()
+        eng.close()
+    assert float((C.cpu().double() - C_ref.detach()).abs().max()) <= 1e-5 * N
+    for name, got, ref in zip(("beta", "sigma", "gamma", "phi"), grads, g_ref):
+        rel = _rel(got.cpu().double(), ref)
+        assert rel <= 1e-4, (name, rel)
+
+
 def test_marginal_error_paths(oracle):
     import ctypes as C
     from replay_amd import ReplayError, load_library

@@ -95,3 +95,27 @@ def test_autograd_matches_central_differences(seed):
             minus[which][idx] -= h
             fd = (loss(*plus) - loss(*minus)).item() / (2 * h)
             assert abs(fd - grads[which][idx].item()) <= 1e-6 * max(1.0, abs(fd)), (which, idx)
+
+
+def test_sparse_reference_equals_dense(oracle):
+    """The sparse-slice FP64 reference (used at full size) gives the dense
+    one's curves and autograd gradients on random graphs, to fp64 rounding."""
+    from test_gpu_parity import _random_case
+    for seed in (3, 4):
+        n, a, b, s, e = _random_case(seed, n=40, e=600, span=20 * 1800, long_frac=0.4)
+        g = oracle.OracleGraph.from_contacts(n, a, b, s, e)
+        p = oracle.make_params(M=1, step_size=1800.0, iterations=20, static_network_duration=3600)
+        dense = marginal_ref.dense_slices(g, p, 20)
+        sparse = marginal_ref.sparse_slices(g, p, 20)
+        rng = np.random.default_rng(seed)
+        xs = [torch.tensor(v, dtype=torch.float64, requires_grad=True) for v in
+              (rng.uniform(0.5, 2, 5), rng.uniform(0.1, 0.5, 5), rng.uniform(0.05, 0.3, 5),
+               rng.uniform(0.5, 1.5, n))]
+        W = torch.tensor(rng.normal(size=(21, 5, 4)))
+        Cd = marginal_ref.simulate(dense, *xs, 0.1, 0.5)
+        gd = torch.autograd.grad((Cd * W).sum(), xs)
+        Cs = marginal_ref.simulate_sparse(sparse, *xs, 0.1, 0.5)
+        gs = torch.autograd.grad((Cs * W).sum(), xs)
+        assert float((Cd - Cs).abs().max()) < 1e-10
+        for x, y in zip(gd, gs):
+            assert float((x - y).abs().max()) <= 1e-10 * max(1.0, float(x.abs().max()))
