@@ -149,6 +149,11 @@ struct replay_ctx {
   // the first long_n_int[e] of them with exact integer sums
   std::vector<int64_t> long_off;
   std::vector<int32_t> long_n_int;
+  // segments of the long rows: epoch e's segments are hub_seg_off[e] ..
+  // hub_seg_off[e+1] of d_hub_seg_row (each segment's long row, relative to
+  // the epoch's list); d_hub_seg0 + long_off[e] + e holds the first segment
+  // of each of the epoch's long rows (relative to the epoch) and their total
+  std::vector<int64_t> hub_seg_off;
   int64_t n_entries = 0;
   int32_t windows_per_epoch = 1;
   bool any_cut = false;
@@ -181,6 +186,11 @@ struct replay_ctx {
   uint32_t *d_rank_limit = nullptr;  // [n_windows][n_net] when a cap bites
   float *d_p_tab = nullptr;          // [n_net][tab_max+1]
   int32_t *d_long_rows = nullptr;    // long rows of every epoch
+  int32_t *d_hub_seg0 = nullptr;
+  int32_t *d_hub_seg_row = nullptr;
+  uint64_t *d_hub_total = nullptr;   // [most long rows of an epoch][W][64]
+  uint32_t *d_hub_part = nullptr;    // [most segments of an epoch][W][64]
+  int64_t hub_total_n = 0;           // elements of d_hub_total
   int32_t tab_max = 0;
   float *d_uniforms = nullptr;
   int cur = 0;

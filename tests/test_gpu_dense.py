@@ -124,6 +124,24 @@ def test_long_rows_match_oracle(oracle, variant):
     np.testing.assert_array_equal(state.reshape(-1), ostate)
 
 
+def test_long_rows_fractional_weights_match_oracle(oracle):
+    """Three 20-minute iterations per one-hour window: weights overlap / 3
+    are not integers, and the hub rows are walked in the reference's fp32
+    order by one wavefront per (row, word)."""
+    n, a, b, s, e = _two_hubs(seed=7, T=12)
+    p = Params(M=96, step_size=1200.0, static_network_duration=3600, iterations=36,
+               initial_infected=0.05, infect_prob=0.2, exposed_duration=1200,
+               infectious_duration=6 * 3600, resistant_duration=10 ** 6, seed=4)
+    with Simulation.from_contacts(n, a, b, s, e, p) as sim:
+        sim.run()
+        counts, state = sim.counts(), sim.state()
+    g = oracle.OracleGraph.from_contacts(n, a, b, s, e)
+    oc, ostate, _ = g.simulate(_to_oracle_params(oracle, p, threads=8), want_state=True)
+    np.testing.assert_array_equal(counts, oc)
+    np.testing.assert_array_equal(state.reshape(-1), ostate)
+    assert oc[-1, :, 1:3].sum() > oc[0, :, 1:3].sum()
+
+
 def test_long_row_sums_beyond_2_24_match_oracle(oracle):
     """The 5000-leaf hub's row is listed beyond the 2^24 bound and is summed
     in integers first.  Here nearly every leaf is seeded infected, so from
@@ -136,6 +154,61 @@ def test_long_row_sums_beyond_2_24_match_oracle(oracle):
                initial_infected=0.97, infect_prob=1e-4, exposed_duration=0,
                infectious_duration=20 * 3600, resistant_duration=10 ** 6, seed=9)
     with Simulation.from_contacts(n, a, b, s, e, p) as sim:
+        sim.run()
+        counts, state = sim.counts(), sim.state()
+    g = oracle.OracleGraph.from_contacts(n, a, b, s, e)
+    oc, ostate, _ = g.simulate(_to_oracle_params(oracle, p, threads=8), want_state=True)
+    np.testing.assert_array_equal(counts, oc)
+    np.testing.assert_array_equal(state.reshape(-1), ostate)
+
+
+def _hub_with_runs(seed=0, leaves=6000, T=10):
+    """One star hub whose leaves each meet it up to three times: one contact
+    spanning the horizon and up to two short ones (partial overlaps), so the
+    hub's row holds runs of entries of one column, spread over several
+    2048-entry segments; leaves also meet each other at random."""
+    rng = np.random.default_rng(seed)
+    n = 1 + leaves
+    t0 = 946713600
+    leaf = np.arange(1, n)
+    extra = rng.integers(0, 3, leaves)
+    short = np.repeat(leaf, extra)
+    ss = t0 + rng.integers(0, T * 3600, short.size)
+    k = 8000
+    a = rng.integers(1, n, k)
+    b = rng.integers(1, n, k)
+    keep = a != b
+    ls = t0 + rng.integers(0, T * 3600, k)
+    src = np.concatenate([np.zeros(leaves + short.size, np.int64), a[keep]]).astype(np.int32)
+    tgt = np.concatenate([leaf, short, b[keep]]).astype(np.int32)
+    st = np.concatenate([np.full(leaves, t0 - 3600), ss, ls[keep]]).astype(np.int64)
+    en = np.concatenate([np.full(leaves, t0 + (T + 2) * 3600),
+                         ss + rng.integers(60, 2 * 3600, short.size),
+                         ls[keep] + rng.integers(60, 4 * 3600, k)[keep]]).astype(np.int64)
+    return n, src, tgt, st, en
+
+
+@pytest.mark.parametrize("M", [128, 200])
+@pytest.mark.parametrize("beta", [1e-4, 0.2])
+def test_hub_runs_beyond_2_24_match_oracle(oracle, M, beta):
+    """A hub row of 6000 leaves, most of them seeded infectious: a
+    susceptible hub replica's sum passes 2^24 s about 80% of the way along
+    the row, in its third segment, and in two of three cases inside a run of
+    entries of one column.  beta = 1e-4 keeps P near 0.4, so the exposure is
+    summed again in the reference's fp32 order from that column on; with
+    beta = 0.2 every exposure of 2^24 s or more gives P = 1 and the draw is
+    settled without it."""
+    n, a, b, s, e = _hub_with_runs(seed=int(M * 10 + beta * 1e4))
+    p = Params(M=M, step_size=3600.0, static_network_duration=3600, iterations=10,
+               initial_infected=0.97, infect_prob=beta, exposed_duration=0,
+               infectious_duration=20 * 3600, resistant_duration=10 ** 6, seed=11)
+    with Simulation.from_contacts(n, a, b, s, e, p) as sim:
+        # the premise: susceptible hub replicas whose exposure passes 2^24 once
+        # the seeds are infectious (iteration 1)
+        sim.run(1)
+        sus = sim.state().reshape(M, n)[:, 0] == 0
+        E1 = sim.exposure(1).reshape(M, n)[:, 0]
+        assert sus.any() and (E1[sus] >= 2 ** 24).any()
         sim.run()
         counts, state = sim.counts(), sim.state()
     g = oracle.OracleGraph.from_contacts(n, a, b, s, e)
