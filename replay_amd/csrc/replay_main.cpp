@@ -13,8 +13,10 @@
 //     start + llround(iter * step) (cu:829-831, 876, 1093-1095);
 //   - the stdout marker line "Starting temporal Monte Carlo simulation for
 //     %d iterations..." (cu:835-836).
-// Differences: --cpu-only and --cpu-threads are accepted for compatibility
-// but the HIP engine always runs; --seed <u64> and --device <n> are added.
+// Differences: the library has no CPU engine, so --cpu-only (cu:1109-1144,
+// the CPU twin) is an error that says so (exit 1) rather than a silent
+// GPU run; --cpu-threads, which only sizes that twin's OpenMP pool, is
+// accepted and has no effect; --seed <u64> and --device <n> are added.
 #include <condition_variable>
 #include <cstdint>
 #include <cstdio>
@@ -115,8 +117,8 @@ class Out {
 void usage(const char *argv0) {
   std::fprintf(stderr, "Usage: %s <graph file> [options]\n", argv0);
   std::fprintf(stderr, "Options:\n");
-  std::fprintf(stderr, "  --cpu-only             Accepted for compatibility; the HIP engine runs.\n");
-  std::fprintf(stderr, "  --cpu-threads <int>    Accepted for compatibility.\n");
+  std::fprintf(stderr, "  --cpu-only             Not supported: this build runs on the GPU only.\n");
+  std::fprintf(stderr, "  --cpu-threads <int>    Accepted; sizes nothing here (no CPU engine).\n");
   std::fprintf(stderr, "  --M <int>              Number of parallel simulations.\n");
   std::fprintf(stderr, "  --step-size <float>    Time step size.\n");
   std::fprintf(stderr, "  --iterations <int>     Number of main simulation steps.\n");
@@ -162,7 +164,10 @@ int main(int argc, char **argv) {
       continue;
     }
     if (!std::strcmp(a, "--cpu-only")) {
-      std::fprintf(stderr, "[INFO] --cpu-only accepted; the HIP engine runs.\n");
+      std::fprintf(stderr,
+                   "Error: --cpu-only is not supported: this build has no CPU engine and runs "
+                   "the simulation on the GPU. Run without --cpu-only.\n");
+      return 1;
     } else if (!std::strcmp(a, "--disable-debug-prints") || !std::strcmp(a, "-q")) {
       quiet = true;
     } else if (!std::strcmp(a, "--cpu-threads") && has) {

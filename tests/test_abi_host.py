@@ -118,6 +118,19 @@ def test_cli_argv_errors(lib, tmp_path):
     assert r.returncode == 1 and "Could not open node state output file" in r.stderr
 
 
+def test_cli_cpu_only_is_refused(lib, tmp_path):
+    """The reference's --cpu-only runs its CPU twin (cu:1109-1144); this
+    build has no CPU engine, so the flag fails loudly instead of handing a
+    caller that times the CPU path a GPU run."""
+    path = _write(tmp_path, "g.txt", "NODE_LIST 1 2\n1 2 2000-01-01T00:00:00 2000-01-01T01:00:00;\n")
+    r = _cli(path, "--cpu-only", "--M", "2", "--iterations", "1")
+    assert r.returncode == 1
+    assert "--cpu-only is not supported" in r.stderr
+    assert "Starting temporal Monte Carlo" not in r.stdout
+    r = _cli("--cpu-threads")  # a value flag without its value is still an argv error
+    assert r.returncode == 1 and "Unrecognized option: --cpu-threads" in r.stderr
+
+
 def test_cli_parse_error_exit_code(lib, tmp_path):
     path = _write(tmp_path, "bad.txt", "NODE_LIST 1 1\n")
     out = tmp_path / "s.csv"
