@@ -241,8 +241,9 @@ __device__ __forceinline__ uint64_t lane_word(uint64_t m, int j) {
 // the entries in order, broadcasting each one's overlap and words from its
 // lane.  en0, en1 are the row's first two chunks of entries, loaded by the
 // caller together with the node's other inputs.  No load sits under an exec
-// mask: lanes without a counting contact read row 0 and are masked after the
-// loads.  Integer mode adds overlaps into exact integer sums;
+// mask: lanes without a counting contact read row col0, a row of the node's
+// own network (its words are at wsh + col0 * W like every neighbour's), and
+// are masked after the loads.  Integer mode adds overlaps into exact integer sums;
 // float mode merges the contributions fl(ov * scale) of one column in rank
 // order and adds columns in ascending order (cu:299-309, Eigen
 // SparseDenseProduct.h:64-70), visiting only entries that overlap the window
@@ -255,8 +256,8 @@ __device__ __forceinline__ uint64_t lane_word(uint64_t m, int j) {
 // holds the sums before that step, and the step's first entry is returned
 // (e1 when no step reaches it).
 template <bool kIntMode, int WT, bool kFull, typename EnT, bool kStop = false>
-__device__ __forceinline__ int group_sums(const StepArgs &a, int64_t wsh, int W, int e0, int e1,
-                                          uint32_t rank_limit, int q0, int nq, int lane,
+__device__ __forceinline__ int group_sums(const StepArgs &a, int64_t wsh, int W, int col0, int e0,
+                                          int e1, uint32_t rank_limit, int q0, int nq, int lane,
                                           const uint64_t (&ss)[WT], const EnT &en0,
                                           const EnT &en1, uint32_t (&iacc)[WT],
                                           float (&facc)[WT]) {
@@ -269,13 +270,13 @@ __device__ __forceinline__ int group_sums(const StepArgs &a, int64_t wsh, int W,
 #pragma unroll
   for (int q = 0; q < WT; ++q) cur_lo[q] = cur_hi[q] = 0;
   // overlap of this lane's entry of chunk c0 (0 when it does not count) and
-  // the column whose words it reads (row 0 when it does not count)
+  // the column whose words it reads (col0 when it does not count)
   auto decode = [&](int c0, const EnT &raw, int &col, int &ov) {
     const EntryFields en = fields(raw);
     int o = min(en.te, a.we_rel) - max(en.ts, a.ws_rel);
     if (!(c0 + lane < e1 && o > 0 && en.rank <= rank_limit)) o = 0;
     ov = o;
-    col = o > 0 ? en.col : 0;
+    col = o > 0 ? en.col : col0;
   };
   // kFull: every network's W is a multiple of WT, so every group is whole
   // and, WT being even, the word offset wsh + col * W + q0 is even (every
@@ -431,9 +432,9 @@ __device__ __forceinline__ int group_sums(const StepArgs &a, int64_t wsh, int W,
 
 // Exposure of the group's words from zero sums (see group_sums).
 template <bool kIntMode, int WT, bool kFull, typename EnT>
-__device__ __forceinline__ void group_exposure(const StepArgs &a, int64_t wsh, int W, int e0,
-                                               int e1, uint32_t rank_limit, int q0, int nq,
-                                               int lane, const uint64_t (&ss)[WT],
+__device__ __forceinline__ void group_exposure(const StepArgs &a, int64_t wsh, int W, int col0,
+                                               int e0, int e1, uint32_t rank_limit, int q0,
+                                               int nq, int lane, const uint64_t (&ss)[WT],
                                                const EnT &en0, const EnT &en1, float (&E)[WT]) {
   uint32_t iacc[WT];
   float facc[WT];
@@ -442,8 +443,8 @@ __device__ __forceinline__ void group_exposure(const StepArgs &a, int64_t wsh, i
     iacc[q] = 0;
     facc[q] = 0.0f;
   }
-  group_sums<kIntMode, WT, kFull, EnT>(a, wsh, W, e0, e1, rank_limit, q0, nq, lane, ss, en0, en1,
-                                       iacc, facc);
+  group_sums<kIntMode, WT, kFull, EnT>(a, wsh, W, col0, e0, e1, rank_limit, q0, nq, lane, ss, en0,
+                                       en1, iacc, facc);
 #pragma unroll
   for (int q = 0; q < WT; ++q) E[q] = kIntMode ? static_cast<float>(iacc[q]) : facc[q];
 }
@@ -460,6 +461,28 @@ constexpr int kTryIntMinEntries = 32;
 __device__ __forceinline__ bool try_integer_sums(const StepArgs &a, int e0, int e1) {
   const int64_t L = static_cast<int64_t>(a.we_rel) - a.ws_rel;
   return L < (int64_t(1) << 24) && static_cast<int64_t>(e1 - e0) * L < (int64_t(1) << 32);
+}
+
+// Timer flips of word w for iteration k+1 (see the header comment): x is
+// the word's infections of iteration k, inf and sus its infectious and
+// susceptible words with the ring's flips at offsets n >= 2 applied.
+__device__ __forceinline__ void apply_flips(const StepArgs &a, const NetDev &nd, int64_t w,
+                                            uint64_t x, uint64_t inf, uint64_t sus) {
+  const int32_t k1 = a.k + 1;
+  if (nd.n_on == 1) inf ^= x;
+  if (nd.n_off == 1) inf ^= x;
+  if (nd.n_s == 1) sus |= x;
+  const bool fl_on = nd.m_on != kNever && nd.m_on >= 1 && k1 == nd.m_on;
+  const bool fl_off = nd.m_off != kNever && nd.m_off >= 1 && k1 == nd.m_off;
+  const bool fl_s = nd.m_s != kNever && nd.m_s >= 1 && k1 == nd.m_s;
+  if (fl_on || fl_off || fl_s) {
+    const uint64_t xi = a.X_init[w];
+    if (fl_on) inf ^= xi;
+    if (fl_off) inf ^= xi;
+    if (fl_s) sus |= xi;
+  }
+  a.I_next[w] = inf;
+  a.S[w] = sus;
 }
 
 // Draw of one word (word Q of node ii) given its lanes' exposure; records the
@@ -609,8 +632,8 @@ void expo_kernel(StepArgs a) {
         // rows of a few contacts keep the float path, which visits only the
         // active entries and costs less there than a column count per word
         if (e1 - e0 > kTryIntMinEntries && try_integer_sums(a, e0, e1)) {
-          group_exposure<true, WT, kFull, EnT>(a, nd.wsh, W, e0, e1, rlim, q0, nq, lane, ss, en0,
-                                               en1, E);
+          group_exposure<true, WT, kFull, EnT>(a, nd.wsh, W, nd.node_off, e0, e1, rlim, q0, nq,
+                                               lane, ss, en0, en1, E);
           bool big = false;
 #pragma unroll
           for (int q = 0; q < WT; ++q)
@@ -619,8 +642,8 @@ void expo_kernel(StepArgs a) {
         }
       }
       if (!done)
-        group_exposure<kIntMode, WT, kFull, EnT>(a, nd.wsh, W, e0, e1, rlim, q0, nq, lane, ss,
-                                                 en0, en1, E);
+        group_exposure<kIntMode, WT, kFull, EnT>(a, nd.wsh, W, nd.node_off, e0, e1, rlim, q0, nq,
+                                                 lane, ss, en0, en1, E);
       uint32_t pw[4] = {0u, 0u, 0u, 0u};
       int pw_grp = -1;
 #pragma unroll
@@ -676,16 +699,16 @@ __global__ __launch_bounds__(kBlock) void long_row_kernel(StepArgs a, const int3
     const v4i en1 = load_chunk(a.entries, e0 + kWave, e1, lane);
     bool done = false;
     if (kTryInt && try_integer_sums(a, e0, e1)) {
-      group_exposure<true, 1, false, v4i>(a, nd.wsh, nd.W, e0, e1, rlim, q, 1, lane, ss, en0,
-                                          en1, E);
+      group_exposure<true, 1, false, v4i>(a, nd.wsh, nd.W, nd.node_off, e0, e1, rlim, q, 1, lane,
+                                          ss, en0, en1, E);
       // E is the integer sum converted to float: >= 2^24 exactly when the
       // sum is
       const bool sus = __builtin_amdgcn_inverse_ballot_w64(ss[0]);
       done = __ballot(sus && E[0] >= 16777216.0f) == 0ull;
     }
     if (!done)
-      group_exposure<kIntMode, 1, false, v4i>(a, nd.wsh, nd.W, e0, e1, rlim, q, 1, lane, ss, en0,
-                                              en1, E);
+      group_exposure<kIntMode, 1, false, v4i>(a, nd.wsh, nd.W, nd.node_off, e0, e1, rlim, q, 1,
+                                              lane, ss, en0, en1, E);
     uint32_t pw[4] = {0u, 0u, 0u, 0u};
     int pw_grp = -1;
     x = draw_word<kInjected>(a, nd, wi, i, q, lane, ss[0], E[0], pw, pw_grp);
@@ -781,8 +804,8 @@ __global__ __launch_bounds__(kBlock) void hub_segment_kernel(StepArgs a, HubArgs
     iacc[q] = 0u;
     facc[q] = 0.0f;
   }
-  group_sums<true, WT, kFull, EnT>(a, nd.wsh, nd.W, c0, c1, rlim, q0, nq, lane, ss, en0, en1,
-                                   iacc, facc);
+  group_sums<true, WT, kFull, EnT>(a, nd.wsh, nd.W, nd.node_off, c0, c1, rlim, q0, nq, lane, ss,
+                                   en0, en1, iacc, facc);
   const bool keep = !saturation_infects(nd.log_base, injected != 0);
 #pragma unroll
   for (int q = 0; q < WT; ++q) {
@@ -856,8 +879,8 @@ __device__ float hub_chain(const StepArgs &a, const HubArgs &h, int hr, int i, c
   float facc[1] = {0.0f};
   v4i en0 = load_chunk(a.entries, cs, ce, lane);
   v4i en1 = ce - cs > kWave ? load_chunk(a.entries, cs + kWave, ce, lane) : v4i{};
-  const int c = group_sums<true, 1, false, v4i, true>(a, nd.wsh, nd.W, cs, ce, rlim, q, 1, lane,
-                                                      ss, en0, en1, iacc, facc);
+  const int c = group_sums<true, 1, false, v4i, true>(a, nd.wsh, nd.W, nd.node_off, cs, ce, rlim,
+                                                      q, 1, lane, ss, en0, en1, iacc, facc);
   // every susceptible replica's sum before entry c is exact and below 2^24;
   // the chain restarts at the first entry of c's column, from the sums
   // before that column
@@ -874,8 +897,8 @@ __device__ float hub_chain(const StepArgs &a, const HubArgs &h, int hr, int i, c
   if (p < e1) {
     en0 = load_chunk(a.entries, p, e1, lane);
     en1 = e1 - p > kWave ? load_chunk(a.entries, p + kWave, e1, lane) : v4i{};
-    group_sums<false, 1, false, v4i>(a, nd.wsh, nd.W, p, e1, rlim, q, 1, lane, ss, en0, en1, iacc,
-                                     facc);
+    group_sums<false, 1, false, v4i>(a, nd.wsh, nd.W, nd.node_off, p, e1, rlim, q, 1, lane, ss,
+                                     en0, en1, iacc, facc);
   }
   return facc[0];
 }
@@ -1247,22 +1270,7 @@ __global__ __launch_bounds__(kBlock) void update_kernel(StepArgs a) {
       const uint64_t fk = __ballot(x != 0ull);
       if (lane == 0) a.F[static_cast<size_t>(a.k % a.depthF) * n_waves + wave] = fk;
     }
-    if (valid) {
-      if (nd.n_on == 1) inf ^= x;
-      if (nd.n_off == 1) inf ^= x;
-      if (nd.n_s == 1) sus |= x;
-      const bool fl_on = nd.m_on != kNever && nd.m_on >= 1 && k1 == nd.m_on;
-      const bool fl_off = nd.m_off != kNever && nd.m_off >= 1 && k1 == nd.m_off;
-      const bool fl_s = nd.m_s != kNever && nd.m_s >= 1 && k1 == nd.m_s;
-      if (fl_on || fl_off || fl_s) {
-        const uint64_t xi = a.X_init[w];
-        if (fl_on) inf ^= xi;
-        if (fl_off) inf ^= xi;
-        if (fl_s) sus |= xi;
-      }
-      a.I_next[w] = inf;
-      a.S[w] = sus;
-    }
+    if (valid) apply_flips(a, nd, w, x, inf, sus);
   }
   const int64_t g = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
   const int64_t G = a.G;
@@ -1799,9 +1807,9 @@ void launch_long(const StepArgs &a, bool int_mode, const LongRows &lr, hipStream
   }
 }
 
-// One iteration: exposure + draws per node (long rows by long_row_kernel),
-// then the word flips.  nb_planes > 0 selects the bit-sliced exposure kernel
-// (integer windows only).
+// One iteration: exposure + draws per node (long rows by the hub kernels or
+// long_row_kernel), then the word flips.  nb_planes > 0 selects the
+// bit-sliced exposure kernel (integer windows only).
 void launch_step(const StepArgs &a, int variant, bool int_mode, bool injected, dim3 grid,
                  hipStream_t st, int nb_planes, const LongRows &lr) {
   if (variant == REPLAY_V_SLICED && !(int_mode && nb_planes > 0))
