@@ -733,7 +733,12 @@ __global__ __launch_bounds__(kBlock) void long_row_kernel(StepArgs a, const int3
 // replica's sum first reaches 2^24, backs up to the first entry of the
 // column the step starts in, and runs the reference's fp32 chain from there,
 // starting from the exact integer sums before it.
-constexpr int kHubSeg = 2048;  // entries per segment
+#ifndef REPLAY_HUB_SEGMENT
+#define REPLAY_HUB_SEGMENT 2048
+#endif
+// entries per segment (1024 and 4096 measured on the dense workload: 1,032
+// and 1,062 ms per run against 1,036)
+constexpr int kHubSeg = REPLAY_HUB_SEGMENT;
 
 struct HubArgs {
   const int32_t *rows;     // the epoch's long rows
