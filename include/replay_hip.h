@@ -122,6 +122,9 @@ typedef struct replay_info {
                                  8 (the 8-byte copy) or 16 */
   int32_t replicas_total;     /* replica columns of all networks (the sum of their M) */
   int32_t ring_form;          /* REPLAY_RING_*: how the infection ring is kept */
+  int32_t has_entries8;       /* 1: the context holds the 8-byte copy of the entries
+                                 (what the marginal form streams); in what was
+                                 padding, so the struct's size and layout are unchanged */
   int64_t ring_bytes;         /* device bytes of the infection ring (or its flags) */
   float tune_ms[4];           /* device ms of each candidate in the last replay_autotune,
                                  in the order WORD, NODE4, NODE16, SLICED; -1 where the

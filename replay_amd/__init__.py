@@ -68,6 +68,7 @@ class _Info(C.Structure):
                 ("ring_depth", C.c_int32), ("n_networks", C.c_int32), ("variant", C.c_int32),
                 ("graph_ready", C.c_int32), ("entry_bytes", C.c_int32),
                 ("replicas_total", C.c_int32), ("ring_form", C.c_int32),
+                ("has_entries8", C.c_int32),
                 ("ring_bytes", C.c_int64), ("tune_ms", C.c_float * 4)]
 
 

@@ -2605,6 +2605,7 @@ int32_t replay_get_info(replay_ctx *ctx, replay_info *out) {
   out->variant = ctx->variant;
   out->graph_ready = ctx->graph != nullptr;
   out->entry_bytes = c8 ? 8 : 16;
+  out->has_entries8 = ctx->d_entries8 != nullptr ? 1 : 0;
   for (int v = 0; v < 4; ++v) out->tune_ms[v] = ctx->tune_ms[v];
   return REPLAY_OK;
 }

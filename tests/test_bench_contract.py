@@ -35,8 +35,10 @@ def test_roofline_arithmetic():
 def test_committed_traffic_summary():
     t = bench.measured_traffic("config3_batched_64nets")
     assert t is not None
-    parts = sum(t[k] for k in ("expo_fetch_bytes", "expo_write_bytes", "update_fetch_bytes",
-                               "update_write_bytes"))
+    # every kernel of the step and the once-per-run tally, per step
+    parts = sum(v for k, v in t.items() if k.endswith(("_fetch_bytes", "_write_bytes")))
+    assert {"expo_fetch_bytes", "expo_write_bytes", "update_fetch_bytes",
+            "update_write_bytes"} <= set(t)
     assert abs(parts - t["hbm_bytes_per_step"]) < 1
     assert "FETCH_SIZE" in t["source"] and "WRITE_SIZE" in t["source"]
 
