@@ -52,9 +52,15 @@ __device__ __forceinline__ void philox4(uint64_t seed, uint32_t c0, uint32_t c1,
     const uint64_t p1 = static_cast<uint64_t>(0xCD9E8D57u) * c2;
     const uint32_t lo0 = static_cast<uint32_t>(p0), hi0 = static_cast<uint32_t>(p0 >> 32);
     const uint32_t lo1 = static_cast<uint32_t>(p1), hi1 = static_cast<uint32_t>(p1 >> 32);
-    c0 = hi1 ^ c1 ^ k0;
+    // Three-way XORs as one v_bitop3_b32 (truth table 0x96) once both
+    // counter words feeding them vary across the wavefront, which they do
+    // from round 4 (c0 from round 3) when only c1 varies, as in the draws.
+    // Earlier, a word that is still wavefront-uniform is combined with the
+    // key by the scalar unit, and one vector XOR finishes the job; a bitop3
+    // there would read two scalar operands, more than one instruction may.
+    c0 = r >= 3 ? __builtin_amdgcn_bitop3_b32(hi1, c1, k0, 0x96) : hi1 ^ c1 ^ k0;
     c1 = lo1;
-    c2 = hi0 ^ c3 ^ k1;
+    c2 = r >= 4 ? __builtin_amdgcn_bitop3_b32(hi0, c3, k1, 0x96) : hi0 ^ c3 ^ k1;
     c3 = lo0;
     k0 += 0x9E3779B9u;
     k1 += 0xBB67AE85u;

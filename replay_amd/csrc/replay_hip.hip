@@ -225,6 +225,76 @@ __device__ __forceinline__ uint32_t column_count(uint64_t x, int lane) {
   return __builtin_amdgcn_ubfe(u, (lane & 32) ? 16u : 0u, 16u);
 }
 
+// The same column count split for groups of four words, so that the last
+// two stages, where fields have room to spare, serve four words at once.
+//
+// count_rows16 runs stages 0-3 of one word's block.  Stages 0 and 1 are
+// written so that the exchanged value feeds a 32-bit add: each lane sends
+// the fields its partner keeps (x >> (w - sh), masked) and keeps its own
+// (x >> sh, masked), so the DPP move folds into the add's operand.  Stage 0's
+// mask m0 is 0x55555555 for rows that count and 0 for rows that do not (one
+// mask per chunk instead of a select per word).  The 64-bit sums are two
+// 32-bit adds: the halves split at a field boundary and no field overflows.
+// The result holds four 8-bit counts: byte j is column 16 j + (lane & 15)
+// summed over the lane's 16-lane row (sel3 = 0x06040200 + 0x01010101 *
+// bit 3 of the lane picks the kept bytes of stage 3, at most 16 each).
+__device__ __forceinline__ uint32_t count_rows16(uint64_t x, uint32_t m0, uint32_t sel3,
+                                                 int lane) {
+  {  // stage 0: 1-bit fields
+    const uint32_t sh = static_cast<uint32_t>(lane) & 1u;
+    const uint64_t keep = x >> sh, send = x >> (1u - sh);
+    const uint32_t lo = row_xor<1>(static_cast<uint32_t>(send) & m0) +
+                        (static_cast<uint32_t>(keep) & m0);
+    const uint32_t hi = row_xor<1>(static_cast<uint32_t>(send >> 32) & m0) +
+                        (static_cast<uint32_t>(keep >> 32) & m0);
+    x = (static_cast<uint64_t>(hi) << 32) | lo;
+  }
+  {  // stage 1: 2-bit fields holding at most 2
+    constexpr uint32_t m = 0x33333333u;
+    const uint32_t sh = static_cast<uint32_t>(lane) & 2u;
+    const uint64_t keep = x >> sh, send = x >> (2u - sh);
+    const uint32_t lo = row_xor<2>(static_cast<uint32_t>(send) & m) +
+                        (static_cast<uint32_t>(keep) & m);
+    const uint32_t hi = row_xor<2>(static_cast<uint32_t>(send >> 32) & m) +
+                        (static_cast<uint32_t>(keep >> 32) & m);
+    x = (static_cast<uint64_t>(hi) << 32) | lo;
+  }
+  {  // stage 2: nibbles hold at most 4, so add first and keep after
+    const uint32_t lo = static_cast<uint32_t>(x), hi = static_cast<uint32_t>(x >> 32);
+    const uint64_t t = (static_cast<uint64_t>(row_xor<4>(hi) + hi) << 32) | (row_xor<4>(lo) + lo);
+    x = (t >> (static_cast<uint32_t>(lane) & 4u)) & 0x0F0F0F0F0F0F0F0Full;
+  }
+  // stage 3: bytes hold at most 8
+  const uint32_t lo = static_cast<uint32_t>(x), hi = static_cast<uint32_t>(x >> 32);
+  return __builtin_amdgcn_perm(row_xor<8>(hi) + hi, row_xor<8>(lo) + lo, sel3);
+}
+
+// Stages 4 and 5 for four words A..D given their count_rows16 results: byte q
+// of the result is the count of column `lane` of word q over all 64 rows.
+//  - stage 4 (lane ^ 16), per pair of words: P = [A0, A2, B0, B2] and
+//    Q = [A1, A3, B1, B3] (byte j of A is Aj); v_permlane16_swap trades the
+//    odd rows of P for the even rows of Q, so the sum holds
+//    [A_b, A_b+2, B_b, B_b+2] for b = bit 4 of the lane, at most 32 each;
+//  - stage 5 (lane ^ 32): U = [A_b, B_b, C_b, D_b], V = [A_b+2, .., D_b+2];
+//    v_permlane32_swap trades the upper half of U for the lower half of V,
+//    so the sum holds index b + 2 * (bit 5 of the lane), i.e. column lane,
+//    of every word, at most 64 each.
+__device__ __forceinline__ uint32_t count_merge4(uint32_t ra, uint32_t rb, uint32_t rc,
+                                                 uint32_t rd) {
+  const auto ab = __builtin_amdgcn_permlane16_swap(__builtin_amdgcn_perm(rb, ra, 0x06040200u),
+                                                   __builtin_amdgcn_perm(rb, ra, 0x07050301u),
+                                                   false, false);
+  const auto cd = __builtin_amdgcn_permlane16_swap(__builtin_amdgcn_perm(rd, rc, 0x06040200u),
+                                                   __builtin_amdgcn_perm(rd, rc, 0x07050301u),
+                                                   false, false);
+  const uint32_t tab = static_cast<uint32_t>(ab[0]) + static_cast<uint32_t>(ab[1]);
+  const uint32_t tcd = static_cast<uint32_t>(cd[0]) + static_cast<uint32_t>(cd[1]);
+  const auto r = __builtin_amdgcn_permlane32_swap(__builtin_amdgcn_perm(tcd, tab, 0x06040200u),
+                                                  __builtin_amdgcn_perm(tcd, tab, 0x07050301u),
+                                                  false, false);
+  return static_cast<uint32_t>(r[0]) + static_cast<uint32_t>(r[1]);
+}
+
 // Lane j's 64-bit word, broadcast to the wavefront (wave-uniform).
 __device__ __forceinline__ uint64_t lane_word(uint64_t m, int j) {
   const uint32_t lo = static_cast<uint32_t>(
@@ -255,6 +325,14 @@ __device__ __forceinline__ uint64_t lane_word(uint64_t m, int j) {
 // chunks) after which a susceptible replica's sum reaches 2^24; iacc then
 // holds the sums before that step, and the step's first entry is returned
 // (e1 when no step reaches it).
+//
+// Groups of a multiple of four words (integer mode, no kStop) take the
+// split column count (count_rows16 per word, count_merge4 per four words).
+// A 4-word group keeps the whole-window counts of the walk in 16-bit fields,
+// two words per register, multiplied by the window length once at the end;
+// a walk covers at most max(kLongRow, kHubSeg) entries, so no field
+// overflows.  Wider groups add each merged count times the window length
+// into iacc at once.
 template <bool kIntMode, int WT, bool kFull, typename EnT, bool kStop = false>
 __device__ __forceinline__ int group_sums(const StepArgs &a, int64_t wsh, int W, int col0, int e0,
                                           int e1, uint32_t rank_limit, int q0, int nq, int lane,
@@ -262,6 +340,12 @@ __device__ __forceinline__ int group_sums(const StepArgs &a, int64_t wsh, int W,
                                           const EnT &en1, uint32_t (&iacc)[WT],
                                           float (&facc)[WT]) {
   static_assert(kIntMode || !kStop, "the stop test reads integer sums");
+  constexpr bool kPacked = kIntMode && !kStop && WT % 4 == 0;
+  // 4-word groups: whole-window counts of words (0, 1) and (2, 3), 16 bits
+  // each (wider groups add each merged count into iacc, which takes fewer
+  // registers than 16-bit fields for all their words)
+  uint32_t cnt[2] = {0u, 0u};
+  const uint32_t sel3 = 0x06040200u + 0x01010101u * ((static_cast<uint32_t>(lane) >> 3) & 1u);
   // float mode: the column being merged, its weight and its words
   int cur_col = -1;
   float cur_w = 0.0f;
@@ -323,24 +407,64 @@ __device__ __forceinline__ int group_sums(const StepArgs &a, int64_t wsh, int W,
       const bool full = L > 0 && ov == L;
       const uint64_t counting = __ballot(ov > 0);
       const uint64_t partial = __ballot(ov > 0 && ov != L);
-#pragma unroll
-      for (int q = 0; q < WT; ++q) {
-        // words without susceptible replicas need no exposure
-        if (q >= nq || ss[q] == 0ull) continue;
-        const uint64_t m = raw[q];
+      const uint32_t m0 = full ? 0x55555555u : 0u;
+      // Word q: `count` (the column count) when enough whole-window contacts
+      // contribute, then the other contacts one at a time.  The word of
+      // contact j is broadcast and read as a lane mask (inverse ballot):
+      // replica lane r tests bit r in one select.
+      auto word = [&](int q, uint64_t m, auto &&count) {
         const uint64_t nz = __ballot(m != 0ull) & counting;
-        if (nz == 0ull) continue;
+        if (nz == 0ull) return;
         // few contributing contacts: add them one at a time
         const bool few = __popcll(nz & ~partial) <= kFewContacts;
-        const uint32_t c = (!few && (nz & ~partial)) ? column_count(full ? m : 0ull, lane) : 0u;
-        // the word of contact j is broadcast and read as a lane mask
-        // (inverse ballot): replica lane r tests bit r in one select
+        if (!few && (nz & ~partial) != 0ull) count();
         for (uint64_t pm = few ? nz : (partial & nz); pm != 0ull; pm &= pm - 1) {
           const int j = __builtin_ctzll(pm);
           const uint32_t ovj = static_cast<uint32_t>(__builtin_amdgcn_readlane(ov, j));
           iacc[q] += __builtin_amdgcn_inverse_ballot_w64(lane_word(m, j)) ? ovj : 0u;
         }
-        iacc[q] += static_cast<uint32_t>(L) * c;
+      };
+      if constexpr (kPacked) {
+        // four words at a time: stage-3 bytes of each counted word (zero
+        // for the others), then the shared last stages
+#pragma unroll
+        for (int g = 0; g < WT / 4; ++g) {
+          uint32_t r3[4];
+          bool counted = false;
+#pragma unroll
+          for (int k = 0; k < 4; ++k) {
+            const int q = 4 * g + k;
+            r3[k] = 0u;
+            // words without susceptible replicas need no exposure
+            if (q >= nq || ss[q] == 0ull) continue;
+            word(q, raw[q], [&] {
+              r3[k] = count_rows16(raw[q], m0, sel3, lane);
+              counted = true;
+            });
+          }
+          if (!counted) continue;
+          const uint32_t u = count_merge4(r3[0], r3[1], r3[2], r3[3]);
+          if constexpr (WT == 4) {
+            // bytes 0, 1 and 2, 3 of u into 16-bit fields
+            cnt[0] += __builtin_amdgcn_perm(0u, u, 0x0C010C00u);
+            cnt[1] += __builtin_amdgcn_perm(0u, u, 0x0C030C02u);
+          } else {
+#pragma unroll
+            for (int k = 0; k < 4; ++k)
+              iacc[4 * g + k] += __umul24(static_cast<uint32_t>(L),
+                                          __builtin_amdgcn_ubfe(u, 8u * k, 8u));
+          }
+        }
+      } else {
+#pragma unroll
+        for (int q = 0; q < WT; ++q) {
+          // words without susceptible replicas need no exposure
+          if (q >= nq || ss[q] == 0ull) continue;
+          const uint64_t m = raw[q];
+          word(q, m, [&] {
+            iacc[q] += static_cast<uint32_t>(L) * column_count(full ? m : 0ull, lane);
+          });
+        }
       }
     } else {
       uint32_t mlo[WT], mhi[WT];
@@ -426,6 +550,14 @@ __device__ __forceinline__ int group_sums(const StepArgs &a, int64_t wsh, int W,
     for (int q = 0; q < WT; ++q)
       if (__builtin_amdgcn_inverse_ballot_w64((static_cast<uint64_t>(cur_hi[q]) << 32) | cur_lo[q]))
         facc[q] += cur_w;
+  }
+  if constexpr (kPacked && WT == 4) {
+    const uint32_t L = static_cast<uint32_t>(a.we_rel - a.ws_rel);
+#pragma unroll
+    for (int g = 0; g < 2; ++g) {
+      iacc[2 * g] += L * (cnt[g] & 0xFFFFu);
+      iacc[2 * g + 1] += L * (cnt[g] >> 16);
+    }
   }
   return e1;
 }
@@ -739,6 +871,8 @@ __global__ __launch_bounds__(kBlock) void long_row_kernel(StepArgs a, const int3
 // entries per segment (1024 and 4096 measured on the dense workload: 1,032
 // and 1,062 ms per run against 1,036)
 constexpr int kHubSeg = REPLAY_HUB_SEGMENT;
+// group_sums keeps a 4-word group's counts of one walk in 16-bit fields
+static_assert(rp::kLongRow < 65536 && kHubSeg < 65536, "a walk's counts must fit 16 bits");
 
 struct HubArgs {
   const int32_t *rows;     // the epoch's long rows
