@@ -85,6 +85,7 @@ struct StepArgs {
   int32_t G;                  // replica columns of all networks (sum of their M)
   int32_t n_net, depth;
   int32_t k;
+  int64_t xk_off;             // (k mod depth) * NW: first word of ring slot k
   int32_t ws_rel, we_rel;
   float scale;
   uint32_t slot;

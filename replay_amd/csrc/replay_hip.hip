@@ -721,7 +721,7 @@ void expo_kernel(StepArgs a) {
   const uint64_t s_first = a.S[min(wi + min(lane, WT - 1), a.NW - 1)];
   const NetDev nd = a.nets[net];
   const int W = a.M != 0 ? a.W : nd.W;
-  uint64_t *X_k = a.X + static_cast<size_t>(a.k % a.depth) * a.NW + wi;
+  uint64_t *X_k = a.X + a.xk_off + wi;
   const uint64_t *S = a.S + wi;
   if (e1 - e0 > a.long_row) return;  // long_row_kernel takes it
   if (!kInjected && e1 <= e0) {
@@ -820,7 +820,7 @@ __global__ __launch_bounds__(kBlock) void long_row_kernel(StepArgs a, const int3
   const NetDev nd = a.nets[net];
   if (q >= nd.W) return;
   const int64_t wi = nd.wsh + static_cast<int64_t>(i) * nd.W;
-  uint64_t *X_k = a.X + static_cast<size_t>(a.k % a.depth) * a.NW + wi;
+  uint64_t *X_k = a.X + a.xk_off + wi;
   const uint64_t ss[1] = {a.S[wi + q]};
   uint64_t x = 0ull;
   if (ss[0] != 0ull) {
@@ -1055,7 +1055,7 @@ __global__ __launch_bounds__(kBlock) void hub_finish_kernel(StepArgs a, HubArgs 
   const NetDev nd = a.nets[net];
   if (q >= nd.W) return;
   const int64_t wi = nd.wsh + static_cast<int64_t>(i) * nd.W;
-  uint64_t *X_k = a.X + static_cast<size_t>(a.k % a.depth) * a.NW + wi;
+  uint64_t *X_k = a.X + a.xk_off + wi;
   const uint64_t ss = a.S[wi + q];
   uint64_t x = 0ull;
   if (ss != 0ull) {
@@ -1235,7 +1235,7 @@ __global__ __launch_bounds__(kBlock) void sliced_kernel(StepArgs a, int G) {
     const float log_base = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(logb_l), slot0));
     const int node_off = __builtin_amdgcn_readlane(off_l, slot0);
     const uint32_t stream = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(stream_l), slot0));
-    uint64_t *X_k = a.X + static_cast<size_t>(a.k % a.depth) * a.NW + static_cast<size_t>(i) * W;
+    uint64_t *X_k = a.X + a.xk_off + static_cast<size_t>(i) * W;
     const int np = __builtin_amdgcn_readfirstlane(n_part[g]);
     const bool tab_here = net == tab_net;
     const float *tab = a.p_tab ? a.p_tab + static_cast<size_t>(net) * (a.tab_max + 1) : nullptr;
@@ -1379,7 +1379,7 @@ __global__ __launch_bounds__(kBlock) void update_kernel(StepArgs a) {
     const int64_t w = static_cast<int64_t>(ub.y) + threadIdx.x;
     const bool valid = w < ub.z;
     const NetDev nd = a.nets[net];
-    const uint64_t x = valid ? a.X[static_cast<size_t>(a.k % a.depth) * a.NW + w] : 0ull;
+    const uint64_t x = valid ? a.X[a.xk_off + w] : 0ull;
     uint64_t inf = valid ? a.I_cur[w] : 0ull;
     uint64_t sus = valid ? a.S[w] & ~x : 0ull;
     const int32_t k1 = a.k + 1;
@@ -1804,6 +1804,7 @@ StepArgs rp::step_args(replay_ctx *ctx, int32_t k) {
   a.n_net = ctx->n_net;
   a.depth = ctx->depth;
   a.k = k;
+  a.xk_off = static_cast<int64_t>(k % ctx->depth) * ctx->NW;
   a.ws_rel = static_cast<int32_t>(ctx->win_ws[win] - t0);
   a.we_rel = static_cast<int32_t>(ctx->win_we[win] - t0);
   a.scale = ctx->win_scale[win];
