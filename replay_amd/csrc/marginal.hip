@@ -29,8 +29,10 @@
 // directions of every contact with the same weight, so A_k is symmetric and
 // the transposed product sum_i A_k[i,j] G_i is row j of the same packed CSR.
 //
-// Layout: state, adjoint and checkpoint planes are [4][N][Bp] fp32 with the
-// parameter sets contiguous (Bp = B rounded up to 64): a wavefront owns a
+// Layout: state and checkpoint planes are [4][N][Bp] fp32, adjoint planes
+// [3][N][Bp] (s, e, iota; the adjoint of r is the same for every node and is
+// kept per set), with the parameter sets contiguous (Bp = B rounded up to
+// 64): a wavefront owns a
 // node, lane = parameter set, and every neighbour row is one coalesced
 // 256-byte load.  Reductions over nodes go through per-wavefront partials
 // in a fixed order, so results do not depend on scheduling.
@@ -68,11 +70,26 @@ constexpr int kRowBatch = 8;  // neighbour rows requested together
 // kRowBatch at a time: their neighbour rows (one coalesced load of the Bp
 // sets each) are requested together and then added in entry order, so the
 // sum is the sequential one, term for term.
+//
+// The row's bounds and query cap come in a RowBounds read by the caller
+// (row_bounds), so that a caller can request them before its other inputs:
+// vector loads complete in order, and waiting for bounds requested after
+// those inputs would wait for the inputs as well.
+struct RowBounds {
+  int e0, e1;
+  uint32_t rlim;
+};
+
+__device__ __forceinline__ RowBounds row_bounds(const MfStep &g, int i) {
+  return RowBounds{g.rowptr[i], g.rowptr[i + 1],
+                   g.rank_limit ? g.rank_limit[g.net_of[i]] : UINT32_MAX};
+}
+
 template <typename EnT>
-__device__ __forceinline__ float row_product(const MfStep &g, int i, const float *v, int Bp,
-                                             int b, int lane) {
-  const int e0 = g.rowptr[i], e1 = g.rowptr[i + 1];
-  const uint32_t rlim = g.rank_limit ? g.rank_limit[g.net_of[i]] : UINT32_MAX;
+__device__ __forceinline__ float row_product(const MfStep &g, const RowBounds &rb, const float *v,
+                                             int Bp, int b, int lane) {
+  const int e0 = rb.e0, e1 = rb.e1;
+  const uint32_t rlim = rb.rlim;
   const auto *src = [&] {
     if constexpr (sizeof(EnT) == 8)
       return g.entries8;
@@ -136,6 +153,18 @@ __global__ __launch_bounds__(kBlock) void mf_prep_kernel(const float *beta, cons
 // as few as with 4-wavefront blocks over a quarter of the wavefronts.
 constexpr int kMfWaves = 16;
 constexpr int kMfBlock = kMfWaves * kWave;
+// Wavefronts of the forward and transposed-product kernels (at most one per
+// node): 8,192, one round of the wavefront slots of 256 CUs at 8 per SIMD,
+// each taking every 8,192nd node with the next node's row bounds requested
+// ahead.  Measured on the 20k-node workload (forward ms per calibration
+// step): 4,096 -> 11.06, 8,192 -> 10.64, 12,288 -> 11.35, 16,384 -> 12.21.
+#ifndef REPLAY_MF_MAX_WAVES
+#define REPLAY_MF_MAX_WAVES 8192
+#endif
+constexpr int kMfMaxWaves = REPLAY_MF_MAX_WAVES;
+// Forward steps whose curve partials are kept before one reduction forms
+// their curves (a reduction per step cost a launch of about 4 us per step).
+constexpr int kPartSteps = 64;
 
 // Adds the four compartment sums of the block's wavefronts in wavefront
 // order and writes them as the block's partial part[block][b][4].
@@ -155,7 +184,9 @@ __device__ __forceinline__ void block_partial(const float (&sum)[4], int b, int 
       part[(static_cast<size_t>(blockIdx.x) * Bp + b) * 4 + c] = acc;
     }
   }
-  __syncthreads();
+  // the rows are written again only for a further chunk of parameter sets;
+  // after the last one the other wavefronts need not wait for the sum
+  if (b - lane + kWave < Bp) __syncthreads();
 }
 
 // Initial state and its curve partials.
@@ -196,11 +227,20 @@ __global__ __launch_bounds__(kMfBlock) void mf_forward_kernel(MfStep g, MfParams
     const int b = cb + lane;
     const float kb = p.kb[b], a = p.ac[b], c = p.ac[p.Bp + b];
     float sum[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+    // the row's bounds, then the node's state and susceptibility, are
+    // requested first, so that the state's round trip overlaps the row
+    // product's (entries, neighbour rows) instead of following it; the next
+    // node's bounds are requested with the state, so that they are in when
+    // its turn comes
+    RowBounds rb = wave < p.N ? row_bounds(g, wave) : RowBounds{0, 0, 0u};
     for (int i = wave; i < p.N; i += n_waves) {
-      const float E = row_product<EnT>(g, i, cur + 2 * NB, p.Bp, b, lane);
       const size_t o = static_cast<size_t>(i) * p.Bp + b;
       const float s = cur[o], e = cur[NB + o], io = cur[2 * NB + o], r = cur[3 * NB + o];
-      const float lam = -expm1f(-(kb * p.phi[i]) * E);
+      const float phi = p.phi[i];
+      const RowBounds rb_next = i + n_waves < p.N ? row_bounds(g, i + n_waves) : rb;
+      const float E = row_product<EnT>(g, rb, cur + 2 * NB, p.Bp, b, lane);
+      rb = rb_next;
+      const float lam = -expm1f(-(kb * phi) * E);
       const float s1 = s * (1.0f - lam);
       const float e1 = e * (1.0f - a) + s * lam;
       const float i1 = io * (1.0f - c) + e * a;
@@ -215,28 +255,33 @@ __global__ __launch_bounds__(kMfBlock) void mf_forward_kernel(MfStep g, MfParams
       sum[2] += i1;
       sum[3] += r1;
     }
-    block_partial(sum, b, p.Bp, part);
+    // (the recomputation in the backward pass needs no curves)
+    if (part) block_partial(sum, b, p.Bp, part);
   }
 }
 
-// curves[b][c] = sum over blocks of part[blk][b][c]: one block per output,
-// strided per-thread sums and a fixed-order tree, so the result does not
-// depend on scheduling.
+// curves[s][b][c] = sum over blocks of part[s][blk][b][c] for the n_steps
+// steps of a group: one block per output, strided per-thread sums and a
+// fixed-order tree, so the result does not depend on scheduling.  The steps'
+// partials lie part_stride floats apart and their curves B * 4 apart.
 __global__ __launch_bounds__(kBlock) void mf_reduce_kernel(const float *part, int n_parts, int Bp,
+                                                           int B, size_t part_stride,
                                                            float *curves) {
   __shared__ float red[kBlock];
-  const int t = blockIdx.x;  // b * 4 + c
+  const int t = blockIdx.x % (B * 4);  // b * 4 + c
+  const int step = blockIdx.x / (B * 4);
   const int b = t >> 2, c = t & 3;
+  const float *ps = part + static_cast<size_t>(step) * part_stride;
   float acc = 0.0f;
   for (int w = threadIdx.x; w < n_parts; w += kBlock)
-    acc += part[(static_cast<size_t>(w) * Bp + b) * 4 + c];
+    acc += ps[(static_cast<size_t>(w) * Bp + b) * 4 + c];
   red[threadIdx.x] = acc;
   __syncthreads();
   for (int h = kBlock / 2; h > 0; h >>= 1) {
     if (static_cast<int>(threadIdx.x) < h) red[threadIdx.x] += red[threadIdx.x + h];
     __syncthreads();
   }
-  if (threadIdx.x == 0) curves[t] = red[0];
+  if (threadIdx.x == 0) curves[static_cast<size_t>(step) * B * 4 + t] = red[0];
 }
 
 __global__ void mf_zero_kernel(float *p, size_t n) {
@@ -254,10 +299,18 @@ __device__ __forceinline__ float wave_sum(float v) {
 // Adjoint phase 1 of step k: transition adjoints, dL/dE and the parameter
 // partials.  adj_in holds the adjoint at k+1 without the curve term, gc the
 // curve gradient of iteration k+1 ([B][4], sets >= B contribute nothing).
+//
+// The adjoint of r is the same for every node (r feeds nothing back): the
+// sum of the R-curve gradients of the later iterations.  It is kept as one
+// value per parameter set, rbar_in at k+1 and rbar_out at k (written by
+// wavefront 0), which is the per-node sequence of fp32 additions without
+// its N-fold plane.
 __global__ __launch_bounds__(kBlock) void mf_adjoint_kernel(MfParams p, const float *st,
                                                             const float *E, const float *adj_in,
+                                                            const float *rbar_in,
                                                             const float *gc, int B, float *adj_out,
-                                                            float *G, float *pg, float *phig) {
+                                                            float *rbar_out, float *G, float *pg,
+                                                            float *phig) {
   const int lane = threadIdx.x & (kWave - 1);
   const int wave = blockIdx.x * kWavesPerBlock + static_cast<int>(threadIdx.x / kWave);
   const int n_waves = gridDim.x * kWavesPerBlock;
@@ -271,11 +324,13 @@ __global__ __launch_bounds__(kBlock) void mf_adjoint_kernel(MfParams p, const fl
       for (int q = 0; q < 4; ++q) g[q] = gc[b * 4 + q];
     }
     float d_beta = 0.0f, d_a = 0.0f, d_c = 0.0f;
+    const float rb = rbar_in[b] + g[3];
+    if (wave == 0) rbar_out[b] = rb;
     for (int i = wave; i < p.N; i += n_waves) {
       const size_t o = static_cast<size_t>(i) * p.Bp + b;
       const float s = st[o], e = st[NB + o], io = st[2 * NB + o];
       const float sb = adj_in[o] + g[0], eb = adj_in[NB + o] + g[1];
-      const float ib = adj_in[2 * NB + o] + g[2], rb = adj_in[3 * NB + o] + g[3];
+      const float ib = adj_in[2 * NB + o] + g[2];
       const float phi = p.phi[i];
       const float kappa = kb * phi;
       const float Ei = E[o];
@@ -286,7 +341,6 @@ __global__ __launch_bounds__(kBlock) void mf_adjoint_kernel(MfParams p, const fl
       adj_out[o] = sb * X + eb * lam;
       adj_out[NB + o] = eb * (1.0f - a) + ib * a;
       adj_out[2 * NB + o] = ib * (1.0f - c) + rb * c;
-      adj_out[3 * NB + o] = rb;
       const float dk = -s * D * Ei * X;
       d_beta += dk * (phi / 3600.0f);
       d_a += e * (ib - eb);
@@ -311,9 +365,15 @@ __global__ __launch_bounds__(kMfBlock) void mf_spmm_adj_kernel(MfStep g, MfParam
   const size_t NB = static_cast<size_t>(p.N) * p.Bp;
   for (int cb = 0; cb < p.Bp; cb += kWave) {
     const int b = cb + lane;
+    // as in the forward kernel: the next row's bounds are requested with
+    // this row's output word, before this row's product
+    RowBounds rb = wave < p.N ? row_bounds(g, wave) : RowBounds{0, 0, 0u};
     for (int j = wave; j < p.N; j += n_waves) {
-      const float t = row_product<EnT>(g, j, G, p.Bp, b, lane);
-      adj_out[2 * NB + static_cast<size_t>(j) * p.Bp + b] += t;
+      float *out = adj_out + 2 * NB + static_cast<size_t>(j) * p.Bp + b;
+      const float prev = *out;
+      const RowBounds rb_next = j + n_waves < p.N ? row_bounds(g, j + n_waves) : rb;
+      *out = prev + row_product<EnT>(g, rb, G, p.Bp, b, lane);
+      rb = rb_next;
     }
   }
 }
@@ -356,9 +416,10 @@ struct replay_marginal {
   float *d_ckpt = nullptr;              // [n_ckpt][4][N][Bp]
   float *d_seg = nullptr;               // [every + 1][4][N][Bp]
   float *d_segE = nullptr;              // [every][N][Bp]
-  float *d_adj[2] = {nullptr, nullptr}; // [4][N][Bp]
+  float *d_adj[2] = {nullptr, nullptr}; // [3][N][Bp]: adjoints of s, e, iota
+  float *d_rbar[2] = {nullptr, nullptr}; // [Bp]: the adjoint of r (the same for every node)
   float *d_G = nullptr;                 // [N][Bp]
-  float *d_part = nullptr;              // [n_blocks][Bp][4]
+  float *d_part = nullptr;              // [kPartSteps][n_blocks][Bp][4]
   float *d_pg = nullptr;                // [n_waves][Bp][3]
   float *d_phig = nullptr;              // [N]
   float *d_kb = nullptr, *d_ac = nullptr, *d_phi = nullptr;
@@ -369,7 +430,7 @@ namespace {
 void mf_release(replay_marginal *m) {
   float *bufs[] = {m->d_st[0], m->d_st[1], m->d_ckpt, m->d_seg,  m->d_segE, m->d_adj[0],
                    m->d_adj[1], m->d_G,    m->d_part, m->d_pg,  m->d_phig, m->d_kb,
-                   m->d_ac,     m->d_phi};
+                   m->d_ac,     m->d_phi,  m->d_rbar[0], m->d_rbar[1]};
   for (float *b : bufs) (void)hipFree(b);
 }
 
@@ -429,16 +490,19 @@ int32_t replay_marginal_create(replay_ctx *ctx, int32_t n_sets, int32_t checkpoi
   m->dt_h = ctx->p.step_size / 3600.0f;
   m->n_waves = std::max(1, std::min(ctx->N, 4096)) / kWavesPerBlock * kWavesPerBlock;
   if (m->n_waves == 0) m->n_waves = kWavesPerBlock;
-  m->f_blocks = std::max(1, std::min(ctx->N, 16384) / kMfWaves);
+  m->f_blocks = std::max(1, std::min(ctx->N, kMfMaxWaves) / kMfWaves);
   const size_t plane = static_cast<size_t>(m->N) * m->Bp;
   int rc = REPLAY_OK;
   if ((rc = dev_alloc(ctx, &m->d_st[0], 4 * plane)) || (rc = dev_alloc(ctx, &m->d_st[1], 4 * plane)) ||
       (rc = dev_alloc(ctx, &m->d_ckpt, 4 * plane * std::max(m->n_ckpt, 1))) ||
       (rc = dev_alloc(ctx, &m->d_seg, 4 * plane * (m->every + 1))) ||
       (rc = dev_alloc(ctx, &m->d_segE, plane * m->every)) ||
-      (rc = dev_alloc(ctx, &m->d_adj[0], 4 * plane)) || (rc = dev_alloc(ctx, &m->d_adj[1], 4 * plane)) ||
+      (rc = dev_alloc(ctx, &m->d_adj[0], 3 * plane)) || (rc = dev_alloc(ctx, &m->d_adj[1], 3 * plane)) ||
+      (rc = dev_alloc(ctx, &m->d_rbar[0], static_cast<size_t>(m->Bp))) ||
+      (rc = dev_alloc(ctx, &m->d_rbar[1], static_cast<size_t>(m->Bp))) ||
       (rc = dev_alloc(ctx, &m->d_G, plane)) ||
-      (rc = dev_alloc(ctx, &m->d_part, static_cast<size_t>(m->f_blocks) * m->Bp * 4)) ||
+      (rc = dev_alloc(ctx, &m->d_part,
+                      static_cast<size_t>(kPartSteps) * m->f_blocks * m->Bp * 4)) ||
       (rc = dev_alloc(ctx, &m->d_pg, static_cast<size_t>(m->n_waves) * m->Bp * 3)) ||
       (rc = dev_alloc(ctx, &m->d_phig, static_cast<size_t>(m->N))) ||
       (rc = dev_alloc(ctx, &m->d_kb, static_cast<size_t>(m->Bp))) ||
@@ -469,19 +533,26 @@ int32_t replay_marginal_forward(replay_marginal *m, const float *beta, const flo
                      m->dt_h);
   hipLaunchKernelGGL(mf_init_kernel, dim3(blocks), dim3(kMfBlock), 0, st, m->d_st[0], ctx->d_nets,
                      ctx->d_net_of, m->d_part, m->N, m->Bp);
+  const size_t part_stride = static_cast<size_t>(n_parts) * m->Bp * 4;
   hipLaunchKernelGGL(mf_reduce_kernel, dim3(red_blocks), dim3(kBlock), 0, st, m->d_part, n_parts,
-                     m->Bp, curves);
+                     m->Bp, m->B, part_stride, curves);
   const MfParams p = mf_params(m);
   int cur = 0;
+  // each step leaves its block partials in slot k mod kPartSteps; one
+  // reduction per group of kPartSteps steps forms their curves
   for (int32_t k = 0; k < m->T; ++k) {
     if (k % m->every == 0)
       HIP_OR_FAIL(ctx, hipMemcpyAsync(m->d_ckpt + static_cast<size_t>(k / m->every) * 4 * plane,
                                       m->d_st[cur], 4 * plane * sizeof(float),
                                       hipMemcpyDeviceToDevice, st));
-    launch_forward(mf_step(ctx, k), p, m->d_st[cur], m->d_st[cur ^ 1], nullptr, m->d_part, blocks,
-                   st);
-    hipLaunchKernelGGL(mf_reduce_kernel, dim3(red_blocks), dim3(kBlock), 0, st, m->d_part, n_parts,
-                       m->Bp, curves + static_cast<size_t>(k + 1) * m->B * 4);
+    launch_forward(mf_step(ctx, k), p, m->d_st[cur], m->d_st[cur ^ 1], nullptr,
+                   m->d_part + static_cast<size_t>(k % kPartSteps) * part_stride, blocks, st);
+    if (k % kPartSteps == kPartSteps - 1 || k == m->T - 1) {
+      const int32_t k0 = k - k % kPartSteps;
+      hipLaunchKernelGGL(mf_reduce_kernel, dim3(red_blocks * static_cast<unsigned>(k - k0 + 1)),
+                         dim3(kBlock), 0, st, m->d_part, n_parts, m->Bp, m->B, part_stride,
+                         curves + static_cast<size_t>(k0 + 1) * m->B * 4);
+    }
     cur ^= 1;
   }
   HIP_OR_FAIL(ctx, hipGetLastError());
@@ -501,7 +572,8 @@ int32_t replay_marginal_backward(replay_marginal *m, const float *grad_curves, f
   const size_t plane = static_cast<size_t>(m->N) * m->Bp;
   const unsigned blocks = static_cast<unsigned>(m->n_waves / kWavesPerBlock);
   const MfParams p = mf_params(m);
-  zero(m->d_adj[0], 4 * plane, st);
+  zero(m->d_adj[0], 3 * plane, st);
+  zero(m->d_rbar[0], static_cast<size_t>(m->Bp), st);
   zero(m->d_pg, static_cast<size_t>(m->n_waves) * m->Bp * 3, st);
   zero(m->d_phig, static_cast<size_t>(m->N), st);
   int a = 0;  // adjoint at k+1 lives in d_adj[a]
@@ -513,15 +585,15 @@ int32_t replay_marginal_backward(replay_marginal *m, const float *grad_curves, f
     for (int32_t k = k0; k < k1; ++k) {
       const size_t j = static_cast<size_t>(k - k0);
       launch_forward(mf_step(ctx, k), p, m->d_seg + j * 4 * plane,
-                     m->d_seg + (j + 1) * 4 * plane, m->d_segE + j * plane, m->d_part,
+                     m->d_seg + (j + 1) * 4 * plane, m->d_segE + j * plane, nullptr,
                      static_cast<unsigned>(m->f_blocks), st);
     }
     for (int32_t k = k1 - 1; k >= k0; --k) {
       const size_t j = static_cast<size_t>(k - k0);
       hipLaunchKernelGGL(mf_adjoint_kernel, dim3(blocks), dim3(kBlock), 0, st, p,
                          m->d_seg + j * 4 * plane, m->d_segE + j * plane, m->d_adj[a],
-                         grad_curves + static_cast<size_t>(k + 1) * m->B * 4, m->B,
-                         m->d_adj[a ^ 1], m->d_G, m->d_pg, m->d_phig);
+                         m->d_rbar[a], grad_curves + static_cast<size_t>(k + 1) * m->B * 4, m->B,
+                         m->d_adj[a ^ 1], m->d_rbar[a ^ 1], m->d_G, m->d_pg, m->d_phig);
       launch_spmm_adj(mf_step(ctx, k), p, m->d_G, m->d_adj[a ^ 1],
                       static_cast<unsigned>(m->f_blocks), st);
       a ^= 1;
