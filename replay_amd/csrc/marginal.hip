@@ -151,7 +151,12 @@ __global__ __launch_bounds__(kBlock) void mf_prep_kernel(const float *beta, cons
 // Blocks of the forward and initial-state kernels: 16 wavefronts, so that
 // many nodes are in flight while the per-block partials of the curves stay
 // as few as with 4-wavefront blocks over a quarter of the wavefronts.
-constexpr int kMfWaves = 16;
+// (forward ms per calibration step on the 20k-node workload with 4, 8 and
+// 16 wavefronts per block: 11.07, 9.47, 8.61)
+#ifndef REPLAY_MF_BLOCK_WAVES
+#define REPLAY_MF_BLOCK_WAVES 16
+#endif
+constexpr int kMfWaves = REPLAY_MF_BLOCK_WAVES;
 constexpr int kMfBlock = kMfWaves * kWave;
 // Wavefronts of the forward and transposed-product kernels (at most one per
 // node): 8,192, one round of the wavefront slots of 256 CUs at 8 per SIMD,
