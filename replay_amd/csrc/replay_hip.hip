@@ -1516,35 +1516,47 @@ struct InitArgs {
 };
 
 // cu:73-90 with a counter-based draw (slot 0): seeds, initial bit words,
-// entries and per-replica seed counts.  blockIdx.y is the word; words past
-// the node's own network's W have nothing to do.
+// entries and per-replica seed counts.  One wavefront per (node, group of
+// four words), blockIdx.y being the group: the four words' replicas 64
+// apart share one Philox block (replica r takes word (r >> 6) & 3 of
+// counter {node, (r >> 8) * 64 + (r & 63), 0, stream}), which each lane
+// computes once.  Groups past the node's own network's words have nothing
+// to do.
 __global__ __launch_bounds__(kBlock) void init_kernel(InitArgs a) {
   const int lane = threadIdx.x & (kWave - 1);
   const int wave = threadIdx.x / kWave;
-  const int q = blockIdx.y;
-  const int r = q * kWave + lane;
+  const int q0 = 4 * static_cast<int>(blockIdx.y);
   const int i = blockIdx.x * kWavesPerBlock + wave;
   if (i >= a.N) return;
   const int net = a.net_of[i];
   const NetDev nd = a.nets[net];
-  if (q >= nd.W) return;
-  const bool rv = r < nd.M;
-  const size_t w = static_cast<size_t>(nd.wsh + static_cast<int64_t>(i) * nd.W + q);
-  bool seed = false;
-  if (rv) {
-    const float u = a.uniforms ? a.uniforms[static_cast<size_t>(r) * a.N + i]
-                               : rp::philox_uniform(a.seed, static_cast<uint32_t>(i - nd.node_off),
-                                                    static_cast<uint32_t>(r), 0u, nd.stream);
-    seed = u < nd.p0;
-  }
-  const uint64_t xi = __ballot(seed);
-  const uint64_t valid = __ballot(rv);
-  a.entry[w * kWave + lane] = seed ? 0 : -1;
-  if (seed) atomicAdd(a.init_count + nd.moff + r, 1);
-  if (lane == 0) {
-    a.X_init[w] = xi;
-    a.S[w] = nd.m_s == 0 ? valid : (valid & ~xi);
-    a.I[w] = nd.m_on == 0 ? xi : 0ull;
+  if (q0 >= nd.W) return;
+  uint32_t pw[4] = {0u, 0u, 0u, 0u};
+  if (!a.uniforms)
+    rp::philox4(a.seed, static_cast<uint32_t>(i - nd.node_off),
+                (static_cast<uint32_t>(q0 >> 2) << 6) | static_cast<uint32_t>(lane), 0u, nd.stream,
+                pw);
+  const int nq = min(4, nd.W - q0);
+  for (int k = 0; k < nq; ++k) {
+    const int q = q0 + k;
+    const int r = q * kWave + lane;
+    const bool rv = r < nd.M;
+    const size_t w = static_cast<size_t>(nd.wsh + static_cast<int64_t>(i) * nd.W + q);
+    bool seed = false;
+    if (rv) {
+      const float u = a.uniforms ? a.uniforms[static_cast<size_t>(r) * a.N + i]
+                                 : rp::u24(k == 0 ? pw[0] : k == 1 ? pw[1] : k == 2 ? pw[2] : pw[3]);
+      seed = u < nd.p0;
+    }
+    const uint64_t xi = __ballot(seed);
+    const uint64_t valid = __ballot(rv);
+    a.entry[w * kWave + lane] = seed ? 0 : -1;
+    if (seed) atomicAdd(a.init_count + nd.moff + r, 1);
+    if (lane == 0) {
+      a.X_init[w] = xi;
+      a.S[w] = nd.m_s == 0 ? valid : (valid & ~xi);
+      a.I[w] = nd.m_on == 0 ? xi : 0ull;
+    }
   }
 }
 
@@ -2037,7 +2049,7 @@ int do_reset(replay_ctx *ctx) {
   a.net_of = ctx->d_net_of;
   a.seed = ctx->p.seed;
   a.N = ctx->N;
-  dim3 grid((ctx->N + kWavesPerBlock - 1) / kWavesPerBlock, ctx->W);
+  dim3 grid((ctx->N + kWavesPerBlock - 1) / kWavesPerBlock, (ctx->W + 3) / 4);
   hipLaunchKernelGGL(init_kernel, grid, dim3(kBlock), 0, ctx->stream, a);
   HIP_OR_FAIL(ctx, hipGetLastError());
   ctx->cur = 0;
