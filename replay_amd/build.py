@@ -4,7 +4,9 @@ and the `replay_gen` input generator.
     python -m replay_amd.build        # or replay_amd.build.build()
 
 Outputs (in-tree, git-ignored): replay_amd/lib/libreplay_hip.so,
-replay_amd/bin/replay.  Compiles every time it is called.
+replay_amd/bin/replay, replay_gen and replay_cgg, and
+replay_amd/lib/libreplay_hosttest.so (host evaluation of ingestion and slice
+packing, for the CPU tests only).  Compiles every time it is called.
 """
 from __future__ import annotations
 
@@ -21,6 +23,9 @@ BUILD = os.path.join(PKG, "_build")
 LIB_DIR = os.path.join(PKG, "lib")
 BIN_DIR = os.path.join(PKG, "bin")
 LIB = os.path.join(LIB_DIR, "libreplay_hip.so")
+# host evaluation of ingestion and slice packing for the CPU tests; not part
+# of the product library or its ABI
+HOSTTEST_LIB = os.path.join(LIB_DIR, "libreplay_hosttest.so")
 CLI = os.path.join(BIN_DIR, "replay")
 GEN_CLI = os.path.join(BIN_DIR, "replay_gen")
 CGG_CLI = os.path.join(BIN_DIR, "replay_cgg")
@@ -52,10 +57,16 @@ def build(quiet: bool = True) -> str:
     os.makedirs(LIB_DIR, exist_ok=True)
     os.makedirs(BIN_DIR, exist_ok=True)
     host_objs = []
-    for src in ("ingest.cpp", "host_slices.cpp", "generator.cpp", "contactgraph.cpp"):
+    for src in ("ingest.cpp", "generator.cpp", "contactgraph.cpp"):
         obj = os.path.join(BUILD, src.replace(".cpp", ".o"))
         _run(["g++", *CXX_FLAGS, "-c", os.path.join(CSRC, src), "-o", obj], quiet)
         host_objs.append(obj)
+    hs_obj = os.path.join(BUILD, "host_slices.o")
+    _run(["g++", *CXX_FLAGS, "-c", os.path.join(CSRC, "host_slices.cpp"), "-o", hs_obj], quiet)
+    tmp = HOSTTEST_LIB + ".tmp"
+    _run(["g++", "-shared", "-pthread", host_objs[0], hs_obj, "-Wl,--no-undefined", "-o", tmp],
+         quiet)
+    os.replace(tmp, HOSTTEST_LIB)
     hip_objs = []
     for src in ("replay_hip.hip", "marginal.hip"):
         obj = os.path.join(BUILD, src.replace(".hip", ".o"))

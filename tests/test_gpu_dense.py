@@ -241,7 +241,8 @@ def test_daily_windows_beyond_2_24_match_oracle(oracle, M, seeded):
                infectious_duration=20 * 86400, resistant_duration=10 ** 7, seed=7,
                start_time=t0)
     # every window has integer weights but is not an integer window
-    L = replay_amd.load_library()
+    import hostlib
+    L = hostlib.load()
     L.rp_host_plan_open.argtypes = [C.c_int32, C.c_int64] + [C.POINTER(C.c_int32)] * 2 + \
         [C.POINTER(C.c_int64)] * 2 + [C.POINTER(replay_amd._Params), C.POINTER(C.c_void_p)]
     L.rp_host_plan_int_mode.argtypes = [C.c_void_p, C.c_int32]

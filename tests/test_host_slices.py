@@ -2,11 +2,11 @@
 the CPU: the packed slices evaluated on the host (same rules as the device
 slice kernel) equal the oracle's per-iteration CSR bit for bit."""
 import ctypes as C
-import os
 
 import numpy as np
 import pytest
 
+import hostlib
 import replay_amd
 from replay_amd import Params
 
@@ -15,10 +15,7 @@ from test_gpu_parity import STEP_WINDOWS, _random_case
 
 @pytest.fixture(scope="module")
 def lib():
-    if not os.path.exists(replay_amd.LIB_PATH):
-        from replay_amd import build
-        build.build()
-    L = replay_amd.load_library()
+    L = hostlib.load()
     P = C.POINTER
     L.rp_host_plan_open.argtypes = [C.c_int32, C.c_int64, P(C.c_int32), P(C.c_int32),
                                     P(C.c_int64), P(C.c_int64), P(replay_amd._Params),

@@ -12,7 +12,7 @@ import os
 import numpy as np
 import pytest
 
-import replay_amd
+import hostlib
 from conftest import GRAPHS
 
 FIXTURES = sorted(f for f in os.listdir(GRAPHS) if f.endswith(".txt"))
@@ -20,10 +20,7 @@ FIXTURES = sorted(f for f in os.listdir(GRAPHS) if f.endswith(".txt"))
 
 @pytest.fixture(scope="module")
 def lib():
-    if not os.path.exists(replay_amd.LIB_PATH):
-        from replay_amd import build
-        build.build()
-    L = replay_amd.load_library()
+    L = hostlib.load()
     P = C.POINTER
     L.rp_host_parse_line.argtypes = [C.c_char_p, C.c_int64, C.c_int32, P(C.c_int32),
                                      P(C.c_int32), P(C.c_int64), P(C.c_int64), P(C.c_int32)]
