@@ -144,6 +144,9 @@ typedef struct replay_info {
 #define REPLAY_V_NODE16 16    /* one pass per 16 words, accumulated in registers */
 #define REPLAY_V_SLICED 1     /* one lane per (node, word): bit-sliced contact counts,
                                  whole-window slices only (others use NODE4/NODE16) */
+#define REPLAY_V_BLOCK1 0x100 /* or'ed with NODE4 or NODE16: the exposure kernel in blocks
+                                 of one wavefront instead of four (a schedule; the results
+                                 are the same) */
 
 typedef struct replay_ctx replay_ctx;
 

@@ -79,6 +79,10 @@ F_NO_HISTORY = 8
 RING_OFFSETS, RING_COMPACT, RING_HISTORY = 0, 1, 2
 HISTORY_MAX_ITERATIONS = 1024  # longest run the library keeps in the history form
 V_WORD, V_NODE4, V_NODE16, V_SLICED = 0, 4, 16, 1
+# or'ed with V_NODE4 / V_NODE16: the exposure kernel in blocks of one wavefront
+V_BLOCK1 = 0x100
+# every variant and schedule replay_set_variant accepts (SLICED only for a uniform M)
+VARIANTS_ALL = (V_WORD, V_NODE4, V_NODE4 | V_BLOCK1, V_NODE16, V_NODE16 | V_BLOCK1, V_SLICED)
 
 
 EXPORTED_SYMBOLS = (

@@ -703,7 +703,8 @@ void expo_kernel(StepArgs a) {
   // the bitmap that stays in its own L2.  gridDim.x is a multiple of 8.
   const unsigned per_xcd = gridDim.x / kXcds;
   const unsigned lb = (blockIdx.x % kXcds) * per_xcd + blockIdx.x / kXcds;
-  const int i = static_cast<int>(lb) * kWavesPerBlock +
+  // blocks of four wavefronts, or of one (REPLAY_V_BLOCK1)
+  const int i = static_cast<int>(lb) * static_cast<int>(blockDim.x / kWave) +
                 __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
   if (i >= a.N) return;
   // The node's inputs are requested together before any is waited for: the
@@ -1828,8 +1829,13 @@ namespace {
 
 template <bool kInt, bool kInj, bool kTry>
 void launch_expo_t(const StepArgs &a, int variant, hipStream_t st) {
-  const unsigned nb = (a.N + kWavesPerBlock - 1) / kWavesPerBlock;
+  // REPLAY_V_BLOCK1: one wavefront per block, which refills wavefront slots
+  // as each wavefront ends instead of four at a time (the autotuner picks)
+  const int waves = (variant & REPLAY_V_BLOCK1) ? 1 : kWavesPerBlock;
+  variant &= ~REPLAY_V_BLOCK1;
+  const unsigned nb = static_cast<unsigned>((a.N + waves - 1) / waves);
   const dim3 grid((nb + kXcds - 1) / kXcds * kXcds);
+  const dim3 block(static_cast<unsigned>(waves * kWave));
   // whole groups (W a multiple of the group width) take the kernel whose
   // neighbour-word loads are 16 bytes wide
   // and, where the context holds the 8-byte copy of the entries, the one
@@ -1838,20 +1844,20 @@ void launch_expo_t(const StepArgs &a, int variant, hipStream_t st) {
   // (every network's W a multiple of the group width: a.Wg, their gcd)
   if (variant == REPLAY_V_NODE16) {
     if (a.Wg % 16 == 0 && c8)
-      hipLaunchKernelGGL((expo_kernel<kInt, kInj, 16, true, true, kTry>), grid, dim3(kBlock), 0, st, a);
+      hipLaunchKernelGGL((expo_kernel<kInt, kInj, 16, true, true, kTry>), grid, block, 0, st, a);
     else if (a.Wg % 16 == 0)
-      hipLaunchKernelGGL((expo_kernel<kInt, kInj, 16, true, false, kTry>), grid, dim3(kBlock), 0, st, a);
+      hipLaunchKernelGGL((expo_kernel<kInt, kInj, 16, true, false, kTry>), grid, block, 0, st, a);
     else
-      hipLaunchKernelGGL((expo_kernel<kInt, kInj, 16, false, false, kTry>), grid, dim3(kBlock), 0, st, a);
+      hipLaunchKernelGGL((expo_kernel<kInt, kInj, 16, false, false, kTry>), grid, block, 0, st, a);
   } else if (variant == REPLAY_V_NODE4) {
     if (a.Wg % 4 == 0 && c8)
-      hipLaunchKernelGGL((expo_kernel<kInt, kInj, 4, true, true, kTry>), grid, dim3(kBlock), 0, st, a);
+      hipLaunchKernelGGL((expo_kernel<kInt, kInj, 4, true, true, kTry>), grid, block, 0, st, a);
     else if (a.Wg % 4 == 0)
-      hipLaunchKernelGGL((expo_kernel<kInt, kInj, 4, true, false, kTry>), grid, dim3(kBlock), 0, st, a);
+      hipLaunchKernelGGL((expo_kernel<kInt, kInj, 4, true, false, kTry>), grid, block, 0, st, a);
     else
-      hipLaunchKernelGGL((expo_kernel<kInt, kInj, 4, false, false, kTry>), grid, dim3(kBlock), 0, st, a);
+      hipLaunchKernelGGL((expo_kernel<kInt, kInj, 4, false, false, kTry>), grid, block, 0, st, a);
   } else {
-    hipLaunchKernelGGL((expo_kernel<kInt, kInj, 1, false, false, kTry>), grid, dim3(kBlock), 0, st, a);
+    hipLaunchKernelGGL((expo_kernel<kInt, kInj, 1, false, false, kTry>), grid, block, 0, st, a);
   }
 }
 
@@ -1997,6 +2003,7 @@ dim3 step_grid(const replay_ctx *ctx) {
 }
 
 bool variant_applies(const replay_ctx *ctx, int v) {
+  if (v == (REPLAY_V_NODE4 | REPLAY_V_BLOCK1) || v == (REPLAY_V_NODE16 | REPLAY_V_BLOCK1)) return true;
   if (v == REPLAY_V_SLICED)
     return ctx->uniform && ctx->W <= kWave && ctx->N < (1 << 24) &&
            static_cast<int64_t>(ctx->N) * ctx->W < (int64_t(1) << 32);
@@ -2732,8 +2739,9 @@ int32_t replay_get_info(replay_ctx *ctx, replay_info *out) {
                                            : static_cast<double>(ctx->n_entries) /
                                                  static_cast<double>(ctx->epoch_t0.size());
   // entries at the width the NODE4 / NODE16 kernels of whole groups stream
-  const bool c8 = ctx->d_entries8 && ((ctx->variant == REPLAY_V_NODE4 && ctx->Wg % 4 == 0) ||
-                                      (ctx->variant == REPLAY_V_NODE16 && ctx->Wg % 16 == 0));
+  const int base = ctx->variant & ~REPLAY_V_BLOCK1;
+  const bool c8 = ctx->d_entries8 && ((base == REPLAY_V_NODE4 && ctx->Wg % 4 == 0) ||
+                                      (base == REPLAY_V_NODE16 && ctx->Wg % 16 == 0));
   // compact form: a flag bit per word read per offset and one written, in
   // place of the ring reads (the entry reads of flagged words depend on the
   // epidemic and are not counted)
@@ -2824,12 +2832,17 @@ int32_t replay_autotune(replay_ctx *ctx, int32_t steps, int32_t *variant) {
     ~FlagRestore() { c->p.flags = flags; }
   } restore{ctx, ctx->p.flags};
   ctx->p.flags &= ~REPLAY_F_GRAPH;
-  const int candidates[] = {REPLAY_V_WORD, REPLAY_V_NODE4, REPLAY_V_NODE16, REPLAY_V_SLICED};
+  // each variant, NODE4 and NODE16 also in blocks of one wavefront;
+  // tune_ms[slot] keeps the faster schedule of a variant
+  const int candidates[] = {REPLAY_V_WORD,   REPLAY_V_NODE4,
+                            REPLAY_V_NODE4 | REPLAY_V_BLOCK1, REPLAY_V_NODE16,
+                            REPLAY_V_NODE16 | REPLAY_V_BLOCK1, REPLAY_V_SLICED};
+  const int slot_of[] = {0, 1, 1, 2, 2, 3};
+  for (int c = 0; c < 4; ++c) ctx->tune_ms[c] = -1.0f;
   int best = ctx->variant;
   double best_ms = 1e30;
-  for (int c = 0; c < 4; ++c) {
+  for (int c = 0; c < 6; ++c) {
     const int v = candidates[c];
-    ctx->tune_ms[c] = -1.0f;
     if (!variant_applies(ctx, v) || steps <= 0) continue;
     ctx->variant = v;
     // untimed warm-up: the first launch of a kernel also loads its code
@@ -2840,7 +2853,8 @@ int32_t replay_autotune(replay_ctx *ctx, int32_t steps, int32_t *variant) {
     double ms = 0.0;
     rc = replay_run_timed(ctx, steps, &ms);
     if (rc != REPLAY_OK) return rc;
-    ctx->tune_ms[c] = static_cast<float>(ms);
+    float &t = ctx->tune_ms[slot_of[c]];
+    if (t < 0.0f || static_cast<float>(ms) < t) t = static_cast<float>(ms);
     if (ms < best_ms) {
       best_ms = ms;
       best = v;

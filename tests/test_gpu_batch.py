@@ -90,14 +90,13 @@ def test_variants_agree(M):
     shared = Params(M=M, step_size=600.0, static_network_duration=600, iterations=60, seed=2)
     results = []
     with Simulation.from_networks(nets, shared) as sim:
-        for v in (replay_amd.V_WORD, replay_amd.V_NODE4, replay_amd.V_NODE16, replay_amd.V_SLICED):
+        for v in replay_amd.VARIANTS_ALL:
             sim.set_variant(v)
             sim.reset()
             sim.run()
             results.append((sim.counts(), sim.state()))
         best = sim.autotune(20)
-        assert best in (replay_amd.V_WORD, replay_amd.V_NODE4, replay_amd.V_NODE16,
-                        replay_amd.V_SLICED)
+        assert best in replay_amd.VARIANTS_ALL
         sim.run()
         results.append((sim.counts(), sim.state()))
     for c, s in results[1:]:
@@ -131,8 +130,7 @@ def test_autotune_times_every_variant_under_graph_flag():
         g.run()
         assert g.info()["graph_ready"] == 1
         np.testing.assert_array_equal(g.counts(), ref)
-        assert first in (replay_amd.V_WORD, replay_amd.V_NODE4, replay_amd.V_NODE16,
-                         replay_amd.V_SLICED)
+        assert first in replay_amd.VARIANTS_ALL
 
 
 def test_config3_reduced_batch_vs_plain():
@@ -172,7 +170,7 @@ def test_random_batches_every_variant(oracle, case):
         expected.append(g.simulate(op)[0])
     states = []
     with Simulation.from_networks(nets, shared) as sim:
-        for v in (replay_amd.V_WORD, replay_amd.V_NODE4, replay_amd.V_NODE16, replay_amd.V_SLICED):
+        for v in replay_amd.VARIANTS_ALL:
             sim.set_variant(v)
             sim.reset()
             sim.run()
@@ -223,7 +221,7 @@ def test_uneven_replica_counts_every_variant(oracle, case):
         assert list(np.diff(sim.replica_offsets)) == [nw.M for nw in nets]
         with pytest.raises(replay_amd.ReplayError):
             sim.set_variant(replay_amd.V_SLICED)
-        for v in (replay_amd.V_WORD, replay_amd.V_NODE4, replay_amd.V_NODE16):
+        for v in [v for v in replay_amd.VARIANTS_ALL if v != replay_amd.V_SLICED]:
             sim.set_variant(v)
             for rep in range(2 if flags else 1):  # a graph is captured, then replayed
                 sim.reset()

@@ -190,7 +190,7 @@ def test_compact_ring_random_matches_oracle(oracle, case):
     oc, ostate, _ = g.simulate(_to_oracle_params(oracle, p, threads=8), want_state=True)
     with Simulation.from_contacts(nn, a, b, s, e, p) as sim:
         assert sim.info()["ring_form"] in (replay_amd.RING_COMPACT, replay_amd.RING_OFFSETS)
-        for v in (replay_amd.V_WORD, replay_amd.V_NODE4, replay_amd.V_NODE16, replay_amd.V_SLICED):
+        for v in replay_amd.VARIANTS_ALL:
             sim.set_variant(v)
             sim.reset()
             sim.run()

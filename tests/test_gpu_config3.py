@@ -22,7 +22,7 @@ from test_gpu_parity import _to_oracle_params
 
 pytestmark = pytest.mark.gpu
 
-VARIANTS = (replay_amd.V_WORD, replay_amd.V_NODE4, replay_amd.V_NODE16, replay_amd.V_SLICED)
+VARIANTS = replay_amd.VARIANTS_ALL
 
 
 def test_config3_reduced_recipe_matches_oracle(oracle):

@@ -15,7 +15,7 @@ from test_gpu_parity import _random_case, _to_oracle_params
 
 pytestmark = pytest.mark.gpu
 
-VARIANTS = [None, replay_amd.V_WORD, replay_amd.V_NODE4, replay_amd.V_NODE16, replay_amd.V_SLICED]
+VARIANTS = [None, *replay_amd.VARIANTS_ALL]
 
 
 def _case(i):

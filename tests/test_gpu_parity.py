@@ -172,7 +172,7 @@ def test_every_variant_matches_oracle_wide(oracle, M, step, D, flags):
     counts, state, _ = g.simulate(_to_oracle_params(oracle, p, threads=8), want_state=True)
     assert counts[:, :, 1].max() > 0
     with Simulation.from_contacts(n, a, b, s, e, p) as sim:
-        for v in (replay_amd.V_WORD, replay_amd.V_NODE4, replay_amd.V_NODE16, replay_amd.V_SLICED):
+        for v in replay_amd.VARIANTS_ALL:
             sim.set_variant(v)
             sim.reset()
             sim.run()
@@ -211,7 +211,7 @@ def test_random_sweep_every_variant(oracle, case):
     g = oracle.OracleGraph.from_contacts(nn, a, b, s, en)
     counts, state, _ = g.simulate(_to_oracle_params(oracle, p, threads=8), want_state=True)
     with Simulation.from_contacts(nn, a, b, s, en, p) as sim:
-        for v in (replay_amd.V_WORD, replay_amd.V_NODE4, replay_amd.V_NODE16, replay_amd.V_SLICED):
+        for v in replay_amd.VARIANTS_ALL:
             sim.set_variant(v)
             sim.reset()
             sim.run()
