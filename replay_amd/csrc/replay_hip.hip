@@ -1446,24 +1446,51 @@ __global__ __launch_bounds__(kTallyWaves * kWave) void tally_kernel(StepArgs a,
   const uint64_t *X = a.X + static_cast<size_t>(j % a.depth) * a.NW + nd.woff;
   const int32_t n_nodes = static_cast<int32_t>(nd.nwords / nd.W);
   uint32_t acc[4] = {0u, 0u, 0u, 0u};
-  for (int32_t n0 = wv * kWave; n0 < n_nodes; n0 += kTallyWaves * kWave) {
+  const uint32_t sel3 = 0x06040200u + 0x01010101u * ((static_cast<uint32_t>(lane) >> 3) & 1u);
+  // Two tiles per pass: all eight loads are requested before the first is
+  // used, then each tile is turned in LDS and counted (the split column
+  // count: stages 0-3 per word, 4-5 shared by the four words).
+  constexpr int kStride = kTallyWaves * kWave;
+  for (int32_t n0 = wv * kWave; n0 < n_nodes; n0 += 2 * kStride) {
+    uint64_t v[2][4];
 #pragma unroll
-    for (int u = 0; u < 4; ++u) {  // tile word u * 64 + lane: node / 4, word % 4
-      const int idx = u * kWave + lane;
-      const int32_t node = n0 + (idx >> 2);
-      const int qq = idx & 3;
-      tile[wv][qq][idx >> 2] =
-          (node < n_nodes && qq < nq) ? X[static_cast<int64_t>(node) * nd.W + q0 + qq] : 0ull;
-    }
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    for (int h = 0; h < 2; ++h)
 #pragma unroll
-    for (int qq = 0; qq < 4; ++qq) {
-      const uint64_t x = tile[wv][qq][lane];
-      if (__ballot(x != 0ull) != 0ull) acc[qq] += column_count(x, lane);
+      for (int u = 0; u < 4; ++u) {  // tile word u * 64 + lane: node / 4, word % 4
+        const int idx = u * kWave + lane;
+        const int32_t node = n0 + h * kStride + (idx >> 2);
+        const int qq = idx & 3;
+        v[h][u] = (node < n_nodes && qq < nq) ? X[static_cast<int64_t>(node) * nd.W + q0 + qq]
+                                              : 0ull;
+      }
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const int idx = u * kWave + lane;
+        tile[wv][idx & 3][idx >> 2] = v[h][u];
+      }
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      uint32_t r3[4];
+      bool any = false;
+#pragma unroll
+      for (int qq = 0; qq < 4; ++qq) {
+        const uint64_t x = tile[wv][qq][lane];
+        r3[qq] = 0u;
+        if (__ballot(x != 0ull) != 0ull) {
+          r3[qq] = count_rows16(x, 0x55555555u, sel3, lane);
+          any = true;
+        }
+      }
+      if (any) {
+        const uint32_t c = count_merge4(r3[0], r3[1], r3[2], r3[3]);
+#pragma unroll
+        for (int qq = 0; qq < 4; ++qq) acc[qq] += __builtin_amdgcn_ubfe(c, 8u * qq, 8u);
+      }
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
     }
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
   }
 #pragma unroll
   for (int qq = 0; qq < 4; ++qq) part[wv][qq][lane] = acc[qq];
