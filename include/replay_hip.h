@@ -147,6 +147,9 @@ typedef struct replay_info {
 #define REPLAY_V_BLOCK1 0x100 /* or'ed with NODE4 or NODE16: the exposure kernel in blocks
                                  of one wavefront instead of four (a schedule; the results
                                  are the same) */
+#define REPLAY_V_XCD 0x200    /* or'ed with WORD, NODE4 or NODE16 (and BLOCK1): XCD x takes
+                                 the x-th contiguous eighth of the nodes instead of every
+                                 eighth block (a schedule; the results are the same) */
 
 typedef struct replay_ctx replay_ctx;
 

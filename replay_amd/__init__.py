@@ -81,8 +81,17 @@ HISTORY_MAX_ITERATIONS = 1024  # longest run the library keeps in the history fo
 V_WORD, V_NODE4, V_NODE16, V_SLICED = 0, 4, 16, 1
 # or'ed with V_NODE4 / V_NODE16: the exposure kernel in blocks of one wavefront
 V_BLOCK1 = 0x100
-# every variant and schedule replay_set_variant accepts (SLICED only for a uniform M)
-VARIANTS_ALL = (V_WORD, V_NODE4, V_NODE4 | V_BLOCK1, V_NODE16, V_NODE16 | V_BLOCK1, V_SLICED)
+# or'ed with V_WORD / V_NODE4 / V_NODE16 (and V_BLOCK1): XCD x takes the x-th
+# contiguous eighth of the nodes instead of every eighth block
+V_XCD = 0x200
+# the variants and schedules the parity tests run (replay_set_variant also
+# accepts the other combinations of V_XCD; SLICED only for a uniform M)
+VARIANTS_ALL = (V_WORD, V_NODE4, V_NODE4 | V_BLOCK1, V_NODE16, V_NODE16 | V_BLOCK1, V_SLICED,
+                V_WORD | V_XCD, V_NODE4 | V_XCD, V_NODE16 | V_BLOCK1 | V_XCD)
+# every variant and schedule replay_autotune times
+VARIANTS_TUNED = tuple(v | x for x in (0, V_XCD)
+                       for v in (V_WORD, V_NODE4, V_NODE4 | V_BLOCK1, V_NODE16,
+                                 V_NODE16 | V_BLOCK1)) + (V_SLICED,)
 
 
 EXPORTED_SYMBOLS = (

@@ -697,12 +697,19 @@ void expo_kernel(StepArgs a) {
   // kCompact: the chunks are read from the 8-byte copy of the entries
   using EnT = std::conditional_t<kCompact, v2u, v4i>;
   const int lane = threadIdx.x & (kWave - 1);
-  // Workgroups are dealt to the 8 XCDs round-robin; remap so that XCD x
-  // takes the x-th contiguous eighth of the nodes.  Networks are disjoint
-  // row ranges, so an XCD then gathers neighbour words from a small part of
-  // the bitmap that stays in its own L2.  gridDim.x is a multiple of 8.
-  const unsigned per_xcd = gridDim.x / kXcds;
-  const unsigned lb = (blockIdx.x % kXcds) * per_xcd + blockIdx.x / kXcds;
+  // Workgroups are dealt to the 8 XCDs round-robin.  Block b takes the
+  // b-th run of nodes: the XCDs walk the node range together, each taking
+  // every eighth block, so their work stays even however it varies along
+  // the node range and over a run (measured: XCDs given fixed contiguous
+  // eighths of 64 networks ended a launch up to 20% apart), and the
+  // networks in flight are the same few on every XCD.  With xcd_split
+  // (REPLAY_V_XCD) block b is remapped so that XCD x takes the x-th
+  // contiguous eighth of the nodes instead: where contacts stay within
+  // short runs of node ids (villages), a neighbour word is then fetched
+  // into one XCD's L2 rather than into all of them.  gridDim.x is a
+  // multiple of 8.
+  const unsigned lb = a.xcd_split ? (blockIdx.x % kXcds) * (gridDim.x / kXcds) + blockIdx.x / kXcds
+                                  : blockIdx.x;
   // blocks of four wavefronts, or of one (REPLAY_V_BLOCK1)
   const int i = static_cast<int>(lb) * static_cast<int>(blockDim.x / kWave) +
                 __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
@@ -1845,6 +1852,7 @@ StepArgs rp::step_args(replay_ctx *ctx, int32_t k) {
   a.depth = ctx->depth;
   a.k = k;
   a.xk_off = static_cast<int64_t>(k % ctx->depth) * ctx->NW;
+  a.xcd_split = (ctx->variant & REPLAY_V_XCD) != 0;
   a.ws_rel = static_cast<int32_t>(ctx->win_ws[win] - t0);
   a.we_rel = static_cast<int32_t>(ctx->win_we[win] - t0);
   a.scale = ctx->win_scale[win];
@@ -1859,7 +1867,7 @@ void launch_expo_t(const StepArgs &a, int variant, hipStream_t st) {
   // REPLAY_V_BLOCK1: one wavefront per block, which refills wavefront slots
   // as each wavefront ends instead of four at a time (the autotuner picks)
   const int waves = (variant & REPLAY_V_BLOCK1) ? 1 : kWavesPerBlock;
-  variant &= ~REPLAY_V_BLOCK1;
+  variant &= ~(REPLAY_V_BLOCK1 | REPLAY_V_XCD);  // (the mapping is a.xcd_split)
   const unsigned nb = static_cast<unsigned>((a.N + waves - 1) / waves);
   const dim3 grid((nb + kXcds - 1) / kXcds * kXcds);
   const dim3 block(static_cast<unsigned>(waves * kWave));
@@ -2030,6 +2038,12 @@ dim3 step_grid(const replay_ctx *ctx) {
 }
 
 bool variant_applies(const replay_ctx *ctx, int v) {
+  // the XCD mapping is a schedule of the exposure kernel (not of the
+  // bit-sliced one)
+  if (v & REPLAY_V_XCD) {
+    v &= ~REPLAY_V_XCD;
+    if (v == REPLAY_V_SLICED) return false;
+  }
   if (v == (REPLAY_V_NODE4 | REPLAY_V_BLOCK1) || v == (REPLAY_V_NODE16 | REPLAY_V_BLOCK1)) return true;
   if (v == REPLAY_V_SLICED)
     return ctx->uniform && ctx->W <= kWave && ctx->N < (1 << 24) &&
@@ -2766,7 +2780,7 @@ int32_t replay_get_info(replay_ctx *ctx, replay_info *out) {
                                            : static_cast<double>(ctx->n_entries) /
                                                  static_cast<double>(ctx->epoch_t0.size());
   // entries at the width the NODE4 / NODE16 kernels of whole groups stream
-  const int base = ctx->variant & ~REPLAY_V_BLOCK1;
+  const int base = ctx->variant & ~(REPLAY_V_BLOCK1 | REPLAY_V_XCD);
   const bool c8 = ctx->d_entries8 && ((base == REPLAY_V_NODE4 && ctx->Wg % 4 == 0) ||
                                       (base == REPLAY_V_NODE16 && ctx->Wg % 16 == 0));
   // compact form: a flag bit per word read per offset and one written, in
@@ -2859,8 +2873,9 @@ int32_t replay_autotune(replay_ctx *ctx, int32_t steps, int32_t *variant) {
     ~FlagRestore() { c->p.flags = flags; }
   } restore{ctx, ctx->p.flags};
   ctx->p.flags &= ~REPLAY_F_GRAPH;
-  // each variant, NODE4 and NODE16 also in blocks of one wavefront;
-  // tune_ms[slot] keeps the faster schedule of a variant
+  // each variant, NODE4 and NODE16 also in blocks of one wavefront, and the
+  // exposure kernel's variants with either XCD mapping; tune_ms[slot] keeps
+  // the fastest schedule of a variant
   const int candidates[] = {REPLAY_V_WORD,   REPLAY_V_NODE4,
                             REPLAY_V_NODE4 | REPLAY_V_BLOCK1, REPLAY_V_NODE16,
                             REPLAY_V_NODE16 | REPLAY_V_BLOCK1, REPLAY_V_SLICED};
@@ -2868,8 +2883,8 @@ int32_t replay_autotune(replay_ctx *ctx, int32_t steps, int32_t *variant) {
   for (int c = 0; c < 4; ++c) ctx->tune_ms[c] = -1.0f;
   int best = ctx->variant;
   double best_ms = 1e30;
-  for (int c = 0; c < 6; ++c) {
-    const int v = candidates[c];
+  for (int c = 0; c < 12; ++c) {
+    const int v = candidates[c % 6] | (c >= 6 ? REPLAY_V_XCD : 0);
     if (!variant_applies(ctx, v) || steps <= 0) continue;
     ctx->variant = v;
     // untimed warm-up: the first launch of a kernel also loads its code
@@ -2880,7 +2895,7 @@ int32_t replay_autotune(replay_ctx *ctx, int32_t steps, int32_t *variant) {
     double ms = 0.0;
     rc = replay_run_timed(ctx, steps, &ms);
     if (rc != REPLAY_OK) return rc;
-    float &t = ctx->tune_ms[slot_of[c]];
+    float &t = ctx->tune_ms[slot_of[c % 6]];
     if (t < 0.0f || static_cast<float>(ms) < t) t = static_cast<float>(ms);
     if (ms < best_ms) {
       best_ms = ms;

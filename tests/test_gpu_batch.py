@@ -96,7 +96,7 @@ def test_variants_agree(M):
             sim.run()
             results.append((sim.counts(), sim.state()))
         best = sim.autotune(20)
-        assert best in replay_amd.VARIANTS_ALL
+        assert best in replay_amd.VARIANTS_TUNED
         sim.run()
         results.append((sim.counts(), sim.state()))
     for c, s in results[1:]:
@@ -130,7 +130,7 @@ def test_autotune_times_every_variant_under_graph_flag():
         g.run()
         assert g.info()["graph_ready"] == 1
         np.testing.assert_array_equal(g.counts(), ref)
-        assert first in replay_amd.VARIANTS_ALL
+        assert first in replay_amd.VARIANTS_TUNED
 
 
 def test_config3_reduced_batch_vs_plain():

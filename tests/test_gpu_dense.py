@@ -66,7 +66,7 @@ def test_dense_full_size_invariants():
         assert info["n_nodes"] == 50000 and info["M"] == 4096
         results = []
         for v in (replay_amd.V_NODE16, replay_amd.V_NODE16 | replay_amd.V_BLOCK1,
-                  replay_amd.V_SLICED):
+                  replay_amd.V_NODE16 | replay_amd.V_XCD, replay_amd.V_SLICED):
             sim.set_variant(v)
             sim.reset()
             sim.run()
