@@ -86,10 +86,10 @@ struct StepArgs {
   int32_t n_net, depth;
   int32_t k;
   int64_t xk_off;             // (k mod depth) * NW: first word of ring slot k
-  int32_t xcd_split;          // expo_kernel: XCD x takes the x-th contiguous eighth of the nodes
   int32_t ws_rel, we_rel;
   float scale;
   uint32_t slot;
+  int32_t xcd_split;          // expo_kernel: XCD x takes the x-th contiguous eighth of the nodes
 };
 
 }  // namespace rp

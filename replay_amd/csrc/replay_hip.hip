@@ -707,9 +707,13 @@ void expo_kernel(StepArgs a) {
   // contiguous eighth of the nodes instead: where contacts stay within
   // short runs of node ids (villages), a neighbour word is then fetched
   // into one XCD's L2 rather than into all of them.  gridDim.x is a
-  // multiple of 8.
-  const unsigned lb = a.xcd_split ? (blockIdx.x % kXcds) * (gridDim.x / kXcds) + blockIdx.x / kXcds
-                                  : blockIdx.x;
+  // multiple of 8.  The choice is a mask, not a branch, so that the flag,
+  // the grid size and the block size are requested together (a branch put
+  // two more rounds of scalar loads before the node's first load: 1.2% per
+  // iteration on 10,000 short rows).
+  const unsigned split = 0u - static_cast<unsigned>(a.xcd_split != 0);
+  const unsigned remap = (blockIdx.x % kXcds) * (gridDim.x / kXcds) + blockIdx.x / kXcds;
+  const unsigned lb = (remap & split) | (blockIdx.x & ~split);
   // blocks of four wavefronts, or of one (REPLAY_V_BLOCK1)
   const int i = static_cast<int>(lb) * static_cast<int>(blockDim.x / kWave) +
                 __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
