@@ -333,7 +333,7 @@ __device__ __forceinline__ uint64_t lane_word(uint64_t m, int j) {
 // a walk covers at most max(kLongRow, kHubSeg) entries, so no field
 // overflows.  Wider groups add each merged count times the window length
 // into iacc at once.
-template <bool kIntMode, int WT, bool kFull, typename EnT, bool kStop = false>
+template <bool kIntMode, int WT, bool kFull, typename EnT, bool kStop = false, bool kLive = false>
 __device__ __forceinline__ int group_sums(const StepArgs &a, int64_t wsh, int W, int col0, int e0,
                                           int e1, uint32_t rank_limit, int q0, int nq, int lane,
                                           const uint64_t (&ss)[WT], const EnT &en0,
@@ -435,8 +435,9 @@ __device__ __forceinline__ int group_sums(const StepArgs &a, int64_t wsh, int W,
           for (int k = 0; k < 4; ++k) {
             const int q = 4 * g + k;
             r3[k] = 0u;
-            // words without susceptible replicas need no exposure
-            if (q >= nq || ss[q] == 0ull) continue;
+            // words without susceptible replicas need no exposure (kLive:
+            // the caller found every word of the group live)
+            if (!kLive && (q >= nq || ss[q] == 0ull)) continue;
             word(q, raw[q], [&] {
               r3[k] = count_rows16(raw[q], m0, sel3, lane);
               counted = true;
@@ -459,7 +460,7 @@ __device__ __forceinline__ int group_sums(const StepArgs &a, int64_t wsh, int W,
 #pragma unroll
         for (int q = 0; q < WT; ++q) {
           // words without susceptible replicas need no exposure
-          if (q >= nq || ss[q] == 0ull) continue;
+          if (!kLive && (q >= nq || ss[q] == 0ull)) continue;
           const uint64_t m = raw[q];
           word(q, m, [&] {
             iacc[q] += static_cast<uint32_t>(L) * column_count(full ? m : 0ull, lane);
@@ -563,7 +564,7 @@ __device__ __forceinline__ int group_sums(const StepArgs &a, int64_t wsh, int W,
 }
 
 // Exposure of the group's words from zero sums (see group_sums).
-template <bool kIntMode, int WT, bool kFull, typename EnT>
+template <bool kIntMode, int WT, bool kFull, typename EnT, bool kLive = false>
 __device__ __forceinline__ void group_exposure(const StepArgs &a, int64_t wsh, int W, int col0,
                                                int e0, int e1, uint32_t rank_limit, int q0,
                                                int nq, int lane, const uint64_t (&ss)[WT],
@@ -575,8 +576,8 @@ __device__ __forceinline__ void group_exposure(const StepArgs &a, int64_t wsh, i
     iacc[q] = 0;
     facc[q] = 0.0f;
   }
-  group_sums<kIntMode, WT, kFull, EnT>(a, wsh, W, col0, e0, e1, rank_limit, q0, nq, lane, ss, en0,
-                                       en1, iacc, facc);
+  group_sums<kIntMode, WT, kFull, EnT, false, kLive>(a, wsh, W, col0, e0, e1, rank_limit, q0, nq,
+                                                    lane, ss, en0, en1, iacc, facc);
 #pragma unroll
   for (int q = 0; q < WT; ++q) E[q] = kIntMode ? static_cast<float>(iacc[q]) : facc[q];
 }
@@ -785,9 +786,23 @@ void expo_kernel(StepArgs a) {
           done = __ballot(big) == 0ull;
         }
       }
-      if (!done)
-        group_exposure<kIntMode, WT, kFull, EnT>(a, nd.wsh, W, nd.node_off, e0, e1, rlim, q0, nq,
-                                                 lane, ss, en0, en1, E);
+      if (!done) {
+        // integer sums over whole 16-word groups in which every word has a
+        // susceptible replica (the common case) skip the per-chunk test of
+        // each word.  Measured: village 72.8 -> 68.9 ms and dense 819 ->
+        // 790 ms per run (the 16-word instance also stops spilling VGPRs);
+        // for 4-word groups the second copy of the walk cost 0.4%.
+        constexpr bool kSpecial = kIntMode && kFull && WT == 16;
+        bool live = kSpecial;
+#pragma unroll
+        for (int q = 0; q < WT; ++q) live &= ss[q] != 0ull;
+        if (live)
+          group_exposure<kIntMode, WT, kFull, EnT, kSpecial>(
+              a, nd.wsh, W, nd.node_off, e0, e1, rlim, q0, nq, lane, ss, en0, en1, E);
+        else
+          group_exposure<kIntMode, WT, kFull, EnT>(a, nd.wsh, W, nd.node_off, e0, e1, rlim, q0, nq,
+                                                   lane, ss, en0, en1, E);
+      }
       uint32_t pw[4] = {0u, 0u, 0u, 0u};
       int pw_grp = -1;
 #pragma unroll
