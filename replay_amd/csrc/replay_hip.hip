@@ -225,6 +225,20 @@ __device__ __forceinline__ uint32_t column_count(uint64_t x, int lane) {
   return __builtin_amdgcn_ubfe(u, (lane & 32) ? 16u : 0u, 16u);
 }
 
+// Word of lane (lane ^ 7) within an 8-lane half-row (DPP row_half_mirror).
+__device__ __forceinline__ uint32_t half_row_mirror(uint32_t v) {
+  return static_cast<uint32_t>(__builtin_amdgcn_mov_dpp(static_cast<int>(v), 0x141, 0xF, 0xF, true));
+}
+
+// count_merge4's lane l holds column l ^ (3 * bit 2 of l): lanes with bit 2
+// set (DPP banks 1 and 3 of each row) take the value of lane l ^ 3 (quad
+// reversal); the others keep their own.  Linear, so sums of merged counts
+// are fixed once.
+__device__ __forceinline__ uint32_t count_fix(uint32_t v) {
+  return static_cast<uint32_t>(
+      __builtin_amdgcn_update_dpp(static_cast<int>(v), static_cast<int>(v), 0x1B, 0xF, 0xA, false));
+}
+
 // The same column count split for groups of four words, so that the last
 // two stages, where fields have room to spare, serve four words at once.
 //
@@ -235,13 +249,20 @@ __device__ __forceinline__ uint32_t column_count(uint64_t x, int lane) {
 // mask m0 is 0x55555555 for rows that count and 0 for rows that do not (one
 // mask per chunk instead of a select per word).  The 64-bit sums are two
 // 32-bit adds: the halves split at a field boundary and no field overflows.
-// The result holds four 8-bit counts: byte j is column 16 j + (lane & 15)
-// summed over the lane's 16-lane row (sel3 = 0x06040200 + 0x01010101 *
-// bit 3 of the lane picks the kept bytes of stage 3, at most 16 each).
+//
+// Stage 2 pairs lane l with l ^ 7 (the half-row mirror, one DPP operand)
+// rather than l ^ 4, which DPP reaches only in two moves.  For the two to
+// hold the same columns, stages 0 and 1 keep the columns whose bit 0 is
+// bit 0 ^ bit 2 of the lane, and whose bit 1 is bit 1 ^ bit 2 (their
+// partners l ^ 1, l ^ 2 keep the others, as before).  The result holds
+// four 8-bit counts of the lane's 16-lane row (sel3 = 0x06040200 +
+// 0x01010101 * bit 3 of the lane picks the kept bytes of stage 3, at most
+// 16 each); after count_merge4 lane l holds column l ^ (3 * bit 2 of l),
+// which count_fix moves to lane l.
 __device__ __forceinline__ uint32_t count_rows16(uint64_t x, uint32_t m0, uint32_t sel3,
                                                  int lane) {
   {  // stage 0: 1-bit fields
-    const uint32_t sh = static_cast<uint32_t>(lane) & 1u;
+    const uint32_t sh = static_cast<uint32_t>(lane ^ (lane >> 2)) & 1u;
     const uint64_t keep = x >> sh, send = x >> (1u - sh);
     const uint32_t lo = row_xor<1>(static_cast<uint32_t>(send) & m0) +
                         (static_cast<uint32_t>(keep) & m0);
@@ -251,7 +272,7 @@ __device__ __forceinline__ uint32_t count_rows16(uint64_t x, uint32_t m0, uint32
   }
   {  // stage 1: 2-bit fields holding at most 2
     constexpr uint32_t m = 0x33333333u;
-    const uint32_t sh = static_cast<uint32_t>(lane) & 2u;
+    const uint32_t sh = static_cast<uint32_t>(lane ^ (lane >> 1)) & 2u;
     const uint64_t keep = x >> sh, send = x >> (2u - sh);
     const uint32_t lo = row_xor<2>(static_cast<uint32_t>(send) & m) +
                         (static_cast<uint32_t>(keep) & m);
@@ -261,7 +282,8 @@ __device__ __forceinline__ uint32_t count_rows16(uint64_t x, uint32_t m0, uint32
   }
   {  // stage 2: nibbles hold at most 4, so add first and keep after
     const uint32_t lo = static_cast<uint32_t>(x), hi = static_cast<uint32_t>(x >> 32);
-    const uint64_t t = (static_cast<uint64_t>(row_xor<4>(hi) + hi) << 32) | (row_xor<4>(lo) + lo);
+    const uint64_t t =
+        (static_cast<uint64_t>(half_row_mirror(hi) + hi) << 32) | (half_row_mirror(lo) + lo);
     x = (t >> (static_cast<uint32_t>(lane) & 4u)) & 0x0F0F0F0F0F0F0F0Full;
   }
   // stage 3: bytes hold at most 8
@@ -446,14 +468,16 @@ __device__ __forceinline__ int group_sums(const StepArgs &a, int64_t wsh, int W,
           if (!counted) continue;
           const uint32_t u = count_merge4(r3[0], r3[1], r3[2], r3[3]);
           if constexpr (WT == 4) {
-            // bytes 0, 1 and 2, 3 of u into 16-bit fields
+            // bytes 0, 1 and 2, 3 of u into 16-bit fields (in count_merge4's
+            // lane order, fixed once at the end)
             cnt[0] += __builtin_amdgcn_perm(0u, u, 0x0C010C00u);
             cnt[1] += __builtin_amdgcn_perm(0u, u, 0x0C030C02u);
           } else {
+            const uint32_t uf = count_fix(u);
 #pragma unroll
             for (int k = 0; k < 4; ++k)
               iacc[4 * g + k] += __umul24(static_cast<uint32_t>(L),
-                                          __builtin_amdgcn_ubfe(u, 8u * k, 8u));
+                                          __builtin_amdgcn_ubfe(uf, 8u * k, 8u));
           }
         }
       } else {
@@ -556,8 +580,9 @@ __device__ __forceinline__ int group_sums(const StepArgs &a, int64_t wsh, int W,
     const uint32_t L = static_cast<uint32_t>(a.we_rel - a.ws_rel);
 #pragma unroll
     for (int g = 0; g < 2; ++g) {
-      iacc[2 * g] += L * (cnt[g] & 0xFFFFu);
-      iacc[2 * g + 1] += L * (cnt[g] >> 16);
+      const uint32_t c = count_fix(cnt[g]);
+      iacc[2 * g] += L * (c & 0xFFFFu);
+      iacc[2 * g + 1] += L * (c >> 16);
     }
   }
   return e1;
@@ -1519,7 +1544,7 @@ __global__ __launch_bounds__(kTallyWaves * kWave) void tally_kernel(StepArgs a,
     }
   }
 #pragma unroll
-  for (int qq = 0; qq < 4; ++qq) part[wv][qq][lane] = acc[qq];
+  for (int qq = 0; qq < 4; ++qq) part[wv][qq][lane] = count_fix(acc[qq]);
   __syncthreads();
   if (wv < nq) {  // wavefront qq sums word q0 + qq over the 8 wavefronts
     uint32_t sum = 0;
