@@ -391,13 +391,14 @@ def bench_marginal(args) -> dict:
     bwd = sum(bwd_ms) / len(bwd_ms)
     log(f"marginal: forward {fwd:.2f} ms, backward {bwd:.2f} ms per calibration step; "
         f"loss {loss.item():.4g}")
-    # forward kernel: per step, four state planes read and written (fp32, 64 sets
-    # padded per node), the epoch's entries (8 bytes where the context holds the
-    # 8-byte copy, which the forward kernel then streams) and row pointers
+    # forward kernel: per step, the three state planes s, e, iota read and
+    # written (fp32, 64 sets padded per node; r = 1 - s - e - iota), the epoch's
+    # entries (8 bytes where the context holds the 8-byte copy, which the
+    # forward kernel then streams) and row pointers
     Bp = (B + 63) // 64 * 64
     ent = info["n_entries"] / max(info["n_epochs"], 1)
     eb = 8 if info.get("has_entries8") else 16
-    step_bytes = 32.0 * N * Bp + eb * ent + 4.0 * (N + 1)
+    step_bytes = 24.0 * N * Bp + eb * ent + 4.0 * (N + 1)
     achieved = step_bytes / (fwd / 1e3 / T) / 1e9
     eng.close()
     one.close()
@@ -415,8 +416,9 @@ def bench_marginal(args) -> dict:
                      "kernel": "mf_forward_kernel (+ per-step reduction)",
                      "bytes_per_launch": step_bytes,
                      "us_per_launch": fwd * 1e3 / T,
-                     "bytes_definition": "per iteration: s, e, iota, r read and written as fp32 "
-                                         "for every node and padded parameter set; the slice "
+                     "bytes_definition": "per iteration: s, e, iota read and written as fp32 "
+                                         "for every node and padded parameter set (r = 1 - s "
+                                         "- e - iota is not stored); the slice "
                                          f"epoch's {eb}-byte contact entries and row pointers"},
     }
 

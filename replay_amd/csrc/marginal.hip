@@ -29,7 +29,9 @@
 // directions of every contact with the same weight, so A_k is symmetric and
 // the transposed product sum_i A_k[i,j] G_i is row j of the same packed CSR.
 //
-// Layout: state and checkpoint planes are [4][N][Bp] fp32, adjoint planes
+// Layout: state and checkpoint planes are [3][N][Bp] fp32 (s, e, iota: r
+// never enters the dynamics, and s + e + iota + r = 1 for every node and
+// set, so r is 1 - s - e - iota where its curve is formed), adjoint planes
 // [3][N][Bp] (s, e, iota; the adjoint of r is the same for every node and is
 // kept per set), with the parameter sets contiguous (Bp = B rounded up to
 // 64): a wavefront owns a
@@ -63,6 +65,7 @@ MfStep mf_step(replay_ctx *ctx, int32_t k) {
 }
 
 constexpr int kRowBatch = 8;  // neighbour rows requested together
+constexpr int kMfPlanes = 3;  // state planes per node and set: s, e, iota
 
 // sum_j A_k[row i, j] * v[j][b] in entry order (ascending column).  The
 // wavefront loads 64 of the row's entries at a time (one per lane, from the
@@ -211,7 +214,6 @@ __global__ __launch_bounds__(kMfBlock) void mf_init_kernel(float *st, const NetD
       st[o] = 1.0f - p0;
       st[NB + o] = 0.0f;
       st[2 * NB + o] = p0;
-      st[3 * NB + o] = 0.0f;
       sum[0] += 1.0f - p0;
       sum[2] += p0;
     }
@@ -240,7 +242,7 @@ __global__ __launch_bounds__(kMfBlock) void mf_forward_kernel(MfStep g, MfParams
     RowBounds rb = wave < p.N ? row_bounds(g, wave) : RowBounds{0, 0, 0u};
     for (int i = wave; i < p.N; i += n_waves) {
       const size_t o = static_cast<size_t>(i) * p.Bp + b;
-      const float s = cur[o], e = cur[NB + o], io = cur[2 * NB + o], r = cur[3 * NB + o];
+      const float s = cur[o], e = cur[NB + o], io = cur[2 * NB + o];
       const float phi = p.phi[i];
       const RowBounds rb_next = i + n_waves < p.N ? row_bounds(g, i + n_waves) : rb;
       const float E = row_product<EnT>(g, rb, cur + 2 * NB, p.Bp, b, lane);
@@ -249,11 +251,10 @@ __global__ __launch_bounds__(kMfBlock) void mf_forward_kernel(MfStep g, MfParams
       const float s1 = s * (1.0f - lam);
       const float e1 = e * (1.0f - a) + s * lam;
       const float i1 = io * (1.0f - c) + e * a;
-      const float r1 = r + io * c;
+      const float r1 = 1.0f - s1 - e1 - i1;  // (r + iota c in exact arithmetic)
       next[o] = s1;
       next[NB + o] = e1;
       next[2 * NB + o] = i1;
-      next[3 * NB + o] = r1;
       if (E_out) E_out[o] = E;
       sum[0] += s1;
       sum[1] += e1;
@@ -498,9 +499,10 @@ int32_t replay_marginal_create(replay_ctx *ctx, int32_t n_sets, int32_t checkpoi
   m->f_blocks = std::max(1, std::min(ctx->N, kMfMaxWaves) / kMfWaves);
   const size_t plane = static_cast<size_t>(m->N) * m->Bp;
   int rc = REPLAY_OK;
-  if ((rc = dev_alloc(ctx, &m->d_st[0], 4 * plane)) || (rc = dev_alloc(ctx, &m->d_st[1], 4 * plane)) ||
-      (rc = dev_alloc(ctx, &m->d_ckpt, 4 * plane * std::max(m->n_ckpt, 1))) ||
-      (rc = dev_alloc(ctx, &m->d_seg, 4 * plane * (m->every + 1))) ||
+  if ((rc = dev_alloc(ctx, &m->d_st[0], kMfPlanes * plane)) ||
+      (rc = dev_alloc(ctx, &m->d_st[1], kMfPlanes * plane)) ||
+      (rc = dev_alloc(ctx, &m->d_ckpt, kMfPlanes * plane * std::max(m->n_ckpt, 1))) ||
+      (rc = dev_alloc(ctx, &m->d_seg, kMfPlanes * plane * (m->every + 1))) ||
       (rc = dev_alloc(ctx, &m->d_segE, plane * m->every)) ||
       (rc = dev_alloc(ctx, &m->d_adj[0], 3 * plane)) || (rc = dev_alloc(ctx, &m->d_adj[1], 3 * plane)) ||
       (rc = dev_alloc(ctx, &m->d_rbar[0], static_cast<size_t>(m->Bp))) ||
@@ -547,8 +549,8 @@ int32_t replay_marginal_forward(replay_marginal *m, const float *beta, const flo
   // reduction per group of kPartSteps steps forms their curves
   for (int32_t k = 0; k < m->T; ++k) {
     if (k % m->every == 0)
-      HIP_OR_FAIL(ctx, hipMemcpyAsync(m->d_ckpt + static_cast<size_t>(k / m->every) * 4 * plane,
-                                      m->d_st[cur], 4 * plane * sizeof(float),
+      HIP_OR_FAIL(ctx, hipMemcpyAsync(m->d_ckpt + static_cast<size_t>(k / m->every) * kMfPlanes * plane,
+                                      m->d_st[cur], kMfPlanes * plane * sizeof(float),
                                       hipMemcpyDeviceToDevice, st));
     launch_forward(mf_step(ctx, k), p, m->d_st[cur], m->d_st[cur ^ 1], nullptr,
                    m->d_part + static_cast<size_t>(k % kPartSteps) * part_stride, blocks, st);
@@ -585,18 +587,18 @@ int32_t replay_marginal_backward(replay_marginal *m, const float *grad_curves, f
   for (int32_t sgi = m->n_ckpt - 1; sgi >= 0; --sgi) {
     const int32_t k0 = sgi * m->every, k1 = std::min(k0 + m->every, m->T);
     // recompute the segment's states and exposures from its checkpoint
-    HIP_OR_FAIL(ctx, hipMemcpyAsync(m->d_seg, m->d_ckpt + static_cast<size_t>(sgi) * 4 * plane,
-                                    4 * plane * sizeof(float), hipMemcpyDeviceToDevice, st));
+    HIP_OR_FAIL(ctx, hipMemcpyAsync(m->d_seg, m->d_ckpt + static_cast<size_t>(sgi) * kMfPlanes * plane,
+                                    kMfPlanes * plane * sizeof(float), hipMemcpyDeviceToDevice, st));
     for (int32_t k = k0; k < k1; ++k) {
       const size_t j = static_cast<size_t>(k - k0);
-      launch_forward(mf_step(ctx, k), p, m->d_seg + j * 4 * plane,
-                     m->d_seg + (j + 1) * 4 * plane, m->d_segE + j * plane, nullptr,
+      launch_forward(mf_step(ctx, k), p, m->d_seg + j * kMfPlanes * plane,
+                     m->d_seg + (j + 1) * kMfPlanes * plane, m->d_segE + j * plane, nullptr,
                      static_cast<unsigned>(m->f_blocks), st);
     }
     for (int32_t k = k1 - 1; k >= k0; --k) {
       const size_t j = static_cast<size_t>(k - k0);
       hipLaunchKernelGGL(mf_adjoint_kernel, dim3(blocks), dim3(kBlock), 0, st, p,
-                         m->d_seg + j * 4 * plane, m->d_segE + j * plane, m->d_adj[a],
+                         m->d_seg + j * kMfPlanes * plane, m->d_segE + j * plane, m->d_adj[a],
                          m->d_rbar[a], grad_curves + static_cast<size_t>(k + 1) * m->B * 4, m->B,
                          m->d_adj[a ^ 1], m->d_rbar[a ^ 1], m->d_G, m->d_pg, m->d_phig);
       launch_spmm_adj(mf_step(ctx, k), p, m->d_G, m->d_adj[a ^ 1],
