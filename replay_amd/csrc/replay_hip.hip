@@ -2924,11 +2924,15 @@ int32_t replay_autotune(replay_ctx *ctx, int32_t steps, int32_t *variant) {
                             REPLAY_V_NODE4 | REPLAY_V_BLOCK1, REPLAY_V_NODE16,
                             REPLAY_V_NODE16 | REPLAY_V_BLOCK1, REPLAY_V_SLICED};
   const int slot_of[] = {0, 1, 1, 2, 2, 3};
+  constexpr int kCand = static_cast<int>(sizeof candidates / sizeof candidates[0]);
+  static_assert(sizeof slot_of / sizeof slot_of[0] == kCand, "one slot per candidate");
   for (int c = 0; c < 4; ++c) ctx->tune_ms[c] = -1.0f;
   int best = ctx->variant;
   double best_ms = 1e30;
-  for (int c = 0; c < 12; ++c) {
-    const int v = candidates[c % 6] | (c >= 6 ? REPLAY_V_XCD : 0);
+  // every candidate with plain blocks, then with contiguous XCD eighths
+  // (variant_applies rejects SLICED | REPLAY_V_XCD)
+  for (int c = 0; c < 2 * kCand; ++c) {
+    const int v = candidates[c % kCand] | (c >= kCand ? REPLAY_V_XCD : 0);
     if (!variant_applies(ctx, v) || steps <= 0) continue;
     ctx->variant = v;
     // untimed warm-up: the first launch of a kernel also loads its code
@@ -2939,7 +2943,7 @@ int32_t replay_autotune(replay_ctx *ctx, int32_t steps, int32_t *variant) {
     double ms = 0.0;
     rc = replay_run_timed(ctx, steps, &ms);
     if (rc != REPLAY_OK) return rc;
-    float &t = ctx->tune_ms[slot_of[c % 6]];
+    float &t = ctx->tune_ms[slot_of[c % kCand]];
     if (t < 0.0f || static_cast<float>(ms) < t) t = static_cast<float>(ms);
     if (ms < best_ms) {
       best_ms = ms;
