@@ -137,6 +137,9 @@ struct replay_ctx {
   bool history = false;
   int32_t tallied = 0;
   int32_t rebuilt = 0;  // iterations whose infections entry[] reflects (history ring)
+  // history ring: entry[] holds the reset's seeds (0) and -1 elsewhere; the
+  // reset leaves it to the first rebuild, which state reads start
+  bool entry_ready = true;
   int32_t *d_tally = nullptr;  // tally blocks: network << 16 | first word of 4
   // merged plan and slices
   std::vector<int64_t> iter_time;

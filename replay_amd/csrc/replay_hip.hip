@@ -1572,10 +1572,14 @@ __global__ void prefix_scan_kernel(int32_t *P, const int32_t *inf_count, int32_t
 // seeded, -1 never, as init_kernel leaves it) from iteration j0 up to k by
 // walking X forward over j0 .. k-1: a later infection overwrites an earlier
 // one.  One thread per word.
-__global__ void rebuild_entry_kernel(StepArgs a, int32_t j0, int32_t k) {
+__global__ void rebuild_entry_kernel(StepArgs a, int32_t j0, int32_t k, int32_t init) {
   const int64_t w = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (w >= a.NW) return;
   int32_t *e = a.entry + w * kWave;
+  if (init) {  // the reset's entries: 0 for a seed, -1 otherwise
+    const uint64_t xi = a.X_init[w];
+    for (int r = 0; r < kWave; ++r) e[r] = ((xi >> r) & 1ull) ? 0 : -1;
+  }
   for (int32_t j = j0; j < k; ++j)
     for (uint64_t x = a.X[static_cast<size_t>(j % a.depth) * a.NW + w]; x != 0ull; x &= x - 1)
       e[__builtin_ctzll(x)] = 2 * j + 1;
@@ -1592,6 +1596,7 @@ struct InitArgs {
   const int32_t *net_of;
   uint64_t seed;
   int32_t N;
+  int32_t write_entry;    // 0: entry[] is left to the first rebuild (history ring)
 };
 
 // cu:73-90 with a counter-based draw (slot 0): seeds, initial bit words,
@@ -1629,7 +1634,7 @@ __global__ __launch_bounds__(kBlock) void init_kernel(InitArgs a) {
     }
     const uint64_t xi = __ballot(seed);
     const uint64_t valid = __ballot(rv);
-    a.entry[w * kWave + lane] = seed ? 0 : -1;
+    if (a.write_entry) a.entry[w * kWave + lane] = seed ? 0 : -1;
     if (seed) atomicAdd(a.init_count + nd.moff + r, 1);
     if (lane == 0) {
       a.X_init[w] = xi;
@@ -2135,6 +2140,10 @@ int do_reset(replay_ctx *ctx) {
   a.I = ctx->d_I[0];
   a.X_init = ctx->d_Xinit;
   a.entry = ctx->d_entry;
+  // the history ring rebuilds entry[] when states are read, so the reset
+  // need not write it (826 MB on the 64-network workload)
+  a.write_entry = ctx->history ? 0 : 1;
+  ctx->entry_ready = !ctx->history;
   a.init_count = ctx->d_init_count;
   a.uniforms = ctx->d_uniforms;
   a.nets = ctx->d_nets;
@@ -2636,12 +2645,13 @@ int noalloc_scratch(replay_ctx *ctx, float **ptr) {
 // History ring: derive entry[] (each replica's last infection) from the
 // ring before a state read.
 int rebuild_entries(replay_ctx *ctx) {
-  if (!ctx->history || ctx->rebuilt >= ctx->steps_done) return REPLAY_OK;
+  if (!ctx->history || (ctx->entry_ready && ctx->rebuilt >= ctx->steps_done)) return REPLAY_OK;
   StepArgs a = step_args(ctx, 0);
   const int64_t nb = (ctx->NW + kBlock - 1) / kBlock;
   hipLaunchKernelGGL(rebuild_entry_kernel, dim3(static_cast<unsigned>(nb)), dim3(kBlock), 0,
-                     ctx->stream, a, ctx->rebuilt, ctx->steps_done);
+                     ctx->stream, a, ctx->rebuilt, ctx->steps_done, ctx->entry_ready ? 0 : 1);
   HIP_OR_FAIL(ctx, hipGetLastError());
+  ctx->entry_ready = true;
   ctx->rebuilt = ctx->steps_done;
   return REPLAY_OK;
 }
