@@ -90,6 +90,8 @@ struct StepArgs {
   float scale;
   uint32_t slot;
   int32_t xcd_split;          // expo_kernel: XCD x takes the x-th contiguous eighth of the nodes
+  const int4 *rowinfo;        // epoch's {row start, row end, network, 0} per node, so that
+                              // expo_kernel reads a node's bounds and network in one load
 };
 
 }  // namespace rp
@@ -172,6 +174,7 @@ struct replay_ctx {
   int4 *d_entries = nullptr;
   uint2 *d_entries8 = nullptr;  // 8-byte copy of d_entries, where every entry fits one
   int32_t *d_rowptr = nullptr;
+  int4 *d_rowinfo = nullptr;  // [n_epochs][N] {row start, row end, network, 0}
   uint64_t *d_S = nullptr;
   uint64_t *d_I[2] = {nullptr, nullptr};
   uint64_t *d_X = nullptr;
